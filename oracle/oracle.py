@@ -1,0 +1,228 @@
+"""ctypes wrapper over the oracle C API (oracle_api.h).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, always as the checker, never as the thing
+measured on the GPU. Two backends implement the same API:
+
+  "port"       oracle/liboracle.so, the project's C++ restatement;
+  "reference"  oracle/_ref/libwaslref.so, the reference's own sources
+               compiled by oracle/Makefile (present only where it was built).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATHS = {
+    "port": os.path.join(HERE, "liboracle.so"),
+    "reference": os.path.join(HERE, "_ref", "libwaslref.so"),
+}
+
+PERIODIC, INFLOW, OUTFLOW = 0, 1, 2
+IC_EULER_REGIONS, IC_VORTEX, IC_RAREFACTION, IC_SMOOTH = 0, 1, 2, 3
+IC_SCALAR_PULSE, IC_SCALAR_REGIONS = 10, 11
+
+_libs: dict[str, C.CDLL] = {}
+
+_P = C.c_void_p
+_i64 = C.c_int64
+_d = C.c_double
+_pd = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_pi64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_pi32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_pu8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_pu64 = C.POINTER(C.c_uint64)
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(PATHS[kind])
+
+
+def load(kind: str = "port") -> C.CDLL:
+    if kind in _libs:
+        return _libs[kind]
+    path = PATHS[kind]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"oracle library {path} is not built (make -C oracle)")
+    lib = C.CDLL(path)
+    lib.orc_error.restype = C.c_char_p
+    lib.orc_tri_euler_create.restype = _P
+    lib.orc_tri_euler_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int]
+    lib.orc_line_scalar_create.restype = _P
+    lib.orc_line_scalar_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int,
+                                           _d, C.c_int, _d]
+    lib.orc_destroy.argtypes = [_P]
+    lib.orc_size.restype = _i64
+    lib.orc_size.argtypes = [_P, C.c_int]
+    lib.orc_tri_topology.argtypes = [_P, _pi64, _pd, _pi32, _pi32, _pi64, _pd]
+    lib.orc_line_topology.argtypes = [_P, _pd, _pi32, _pd]
+    lib.orc_tri_trace.argtypes = [_P, _pi32, _pd]
+    lib.orc_stencil_check_count.restype = _i64
+    lib.orc_stencil_check_count.argtypes = [_P]
+    lib.orc_stencil.argtypes = [_P, _pi32, _pd, _pd, _pd]
+    lib.orc_residual.argtypes = [_P, _pd, _pd]
+    lib.orc_limit.argtypes = [_P, _d, _d, _d, _pd, _pu8]
+    lib.orc_stable_dt.argtypes = [_P, _pd, _d, C.POINTER(_d)]
+    lib.orc_step.argtypes = [_P, _d, C.c_int, _d, _d, _d, _pd]
+    lib.orc_mass.argtypes = [_P, _pd, C.POINTER(_d)]
+    lib.orc_project.argtypes = [_P, C.c_int, _pd, _i64, _pd]
+    lib.orc_run.argtypes = [_P, _d, C.c_int, _d, _d, _d, _d, C.c_uint64, _d, C.c_uint64,
+                            C.c_uint64, _pd, _pu64, C.POINTER(_d), _pd, _i64, C.POINTER(_i64)]
+    for name in ("orc_tri_topology", "orc_line_topology", "orc_tri_trace", "orc_stencil",
+                 "orc_residual", "orc_limit", "orc_stable_dt", "orc_step", "orc_mass",
+                 "orc_project", "orc_run"):
+        getattr(lib, name).restype = C.c_int
+    _libs[kind] = lib
+    return lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Case:
+    """One oracle case (mesh + numerics), mirroring the C API."""
+
+    def __init__(self, lib: C.CDLL, handle: int, dim: int):
+        self.lib = lib
+        self.h = handle
+        self.dim = dim
+        self.E = lib.orc_size(handle, 0)
+        self.F = lib.orc_size(handle, 1)
+        self.Np = lib.orc_size(handle, 2)
+        self.V = lib.orc_size(handle, 3)
+        self.P = lib.orc_size(handle, 4)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise OracleError(self.lib.orc_error().decode())
+
+    @property
+    def state_size(self) -> int:
+        return int(self.V * self.Np * self.E)
+
+    def topology(self):
+        if self.dim == 2:
+            en = np.zeros((self.E, 3), np.int64)
+            exy = np.zeros((self.E, 6))
+            fe = np.zeros((self.F, 2), np.int32)
+            fd = np.zeros((self.F, 2), np.int32)
+            fv = np.zeros((self.F, 4), np.int64)
+            fn = np.zeros((self.F, 2))
+            self._check(self.lib.orc_tri_topology(self.h, en, exy, fe, fd, fv, fn))
+            return dict(element_nodes=en, element_xy=exy, face_elements=fe, face_edges=fd,
+                        face_vertices=fv, face_normals=fn)
+        ex = np.zeros((self.E, 2))
+        fe = np.zeros((self.F, 2), np.int32)
+        fn = np.zeros(self.F)
+        self._check(self.lib.orc_line_topology(self.h, ex, fe, fn))
+        return dict(element_x=ex, face_elements=fe, face_normals=fn)
+
+    def trace(self):
+        d = np.zeros(self.F * self.P, np.int32)
+        b = np.zeros(self.F * self.P * self.Np)
+        self._check(self.lib.orc_tri_trace(self.h, d, b))
+        return d.reshape(self.F, self.P), b.reshape(self.F, self.P, self.Np)
+
+    def stencil(self):
+        K = 3 if self.dim == 2 else 2
+        C_ = self.lib.orc_stencil_check_count(self.h)
+        nb = np.zeros(self.E * K, np.int32)
+        tr = np.zeros(self.E * K * self.Np * self.Np)
+        nrm = np.zeros(self.E * 2)
+        cb = np.zeros(max(C_, 1) * self.Np)
+        self._check(self.lib.orc_stencil(self.h, nb, tr, nrm, cb))
+        return (nb.reshape(self.E, K), tr.reshape(self.E, K, self.Np, self.Np),
+                nrm.reshape(self.E, 2), cb[: C_ * self.Np].reshape(C_, self.Np))
+
+    def residual(self, U):
+        U = np.ascontiguousarray(U, np.float64).ravel()
+        out = np.zeros_like(U)
+        self._check(self.lib.orc_residual(self.h, U, out))
+        return out
+
+    def limit(self, U, threshold, density_floor=1e-10, pressure_floor=1e-10):
+        U = np.array(U, np.float64).ravel()
+        flags = np.zeros(self.E, np.uint8)
+        self._check(self.lib.orc_limit(self.h, threshold, density_floor, pressure_floor, U,
+                                       flags))
+        return U, flags
+
+    def stable_dt(self, U, cfl):
+        U = np.ascontiguousarray(U, np.float64).ravel()
+        dt = _d()
+        self._check(self.lib.orc_stable_dt(self.h, U, cfl, C.byref(dt)))
+        return dt.value
+
+    def step(self, U, dt, limiting=False, threshold=1e-3, density_floor=1e-10,
+             pressure_floor=1e-10):
+        U = np.array(U, np.float64).ravel()
+        self._check(self.lib.orc_step(self.h, dt, int(limiting), threshold, density_floor,
+                                      pressure_floor, U))
+        return U
+
+    def mass(self, U):
+        U = np.ascontiguousarray(U, np.float64).ravel()
+        m = _d()
+        self._check(self.lib.orc_mass(self.h, U, C.byref(m)))
+        return m.value
+
+    def project(self, ic, params=()):
+        prm = np.ascontiguousarray(np.asarray(params, np.float64).ravel())
+        if prm.size == 0:
+            prm = np.zeros(1)
+            n = 0
+        else:
+            n = prm.size
+        U = np.zeros(self.state_size)
+        self._check(self.lib.orc_project(self.h, ic, prm, n, U))
+        return U
+
+    def run(self, U, cfl, end_time, max_steps=10 ** 9, history_interval=10 ** 9, limiting=False,
+            threshold=1e-3, density_floor=1e-10, pressure_floor=1e-10, start_time=0.0,
+            start_step=0, history_capacity=4096):
+        U = np.array(U, np.float64).ravel()
+        step = C.c_uint64()
+        time = _d()
+        hist = np.zeros(history_capacity * 4)
+        rows = _i64()
+        self._check(self.lib.orc_run(self.h, cfl, int(limiting), threshold, density_floor,
+                                     pressure_floor, start_time, start_step, end_time, max_steps,
+                                     history_interval, U, C.byref(step), C.byref(time), hist,
+                                     history_capacity, C.byref(rows)))
+        n = min(rows.value, history_capacity)
+        return U, step.value, time.value, hist[: 4 * n].reshape(n, 4)
+
+
+def tri_euler(node_tags, node_xy, triangles, order, gamma=1.4, boundary=PERIODIC,
+              kind="port") -> Case:
+    lib = load(kind)
+    tags = np.ascontiguousarray(node_tags, np.int64)
+    xy = np.ascontiguousarray(node_xy, np.float64).reshape(-1)
+    tri = np.ascontiguousarray(triangles, np.int64).reshape(-1)
+    h = lib.orc_tri_euler_create(tags.size, tags, xy, tri.size // 3, tri, order, gamma, boundary)
+    if not h:
+        raise OracleError(lib.orc_error().decode())
+    return Case(lib, h, 2)
+
+
+def line_scalar(node_tags, node_x, segments, order, wavespeed, left=(PERIODIC, 0.0),
+                right=(PERIODIC, 0.0), kind="port") -> Case:
+    lib = load(kind)
+    tags = np.ascontiguousarray(node_tags, np.int64)
+    x = np.ascontiguousarray(node_x, np.float64)
+    seg = np.ascontiguousarray(segments, np.int64).reshape(-1)
+    h = lib.orc_line_scalar_create(tags.size, tags, x, seg.size // 2, seg, order, wavespeed,
+                                   left[0], left[1], right[0], right[1])
+    if not h:
+        raise OracleError(lib.orc_error().decode())
+    return Case(lib, h, 1)

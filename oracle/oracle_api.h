@@ -1,0 +1,135 @@
+/*
+ * C API shared by the two CPU oracles of the RKDG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (wasl_amd/, include/)
+ * links or calls this; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg do, and only as the checker.
+ *
+ * Two libraries implement this header:
+ *   oracle/liboracle.so        the project's own single-threaded C++
+ *                              restatement of the reference algorithms
+ *                              (oracle/cpu_ref.cpp);
+ *   oracle/_ref/libwaslref.so  the reference's own sources compiled from
+ *                              /root/reference/cpp by oracle/Makefile and
+ *                              driven through oracle/ref_bind.cpp.
+ * Both are built with -ffp-contract=off and no -march, matching the
+ * reference's default CMake build, so their results can be compared bit for
+ * bit.
+ *
+ * Conventions follow the reference:
+ *   Euler state  U[variable][mode][element]   (cpp/include/wasl/dg/modal_layout.hpp:1-22)
+ *   scalar state U[element][mode]             (dg/scalar_advection/line_residual.hpp:14)
+ * Every function returns 0 on success and -1 on failure; orc_error() then
+ * returns the exception text the reference throws.
+ */
+#ifndef WASL_ORACLE_API_H
+#define WASL_ORACLE_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_case orc_case;
+
+/* Boundary kinds (reference BoundaryKind, line_boundary_conditions.hpp:24). */
+enum { ORC_PERIODIC = 0, ORC_INFLOW = 1, ORC_OUTFLOW = 2 };
+
+/* Initial conditions understood by orc_project. */
+enum {
+  ORC_IC_EULER_REGIONS = 0,  /* params: k*10 doubles (shape,lower,upper,cx,cy,r,rho,u,v,p), bridge.cpp:192 */
+  ORC_IC_VORTEX = 1,         /* params: strength, cx, cy (background rho=u=v=p=1) */
+  ORC_IC_RAREFACTION = 2,    /* params: rho, p, speed (velocity speed * r_hat, zero at origin) */
+  ORC_IC_SMOOTH = 3,         /* params: L, then 4 x (a, m, n, phase): rho = 1 + 0.1 sum a sin(2pi(mx+ny)/L+phase), p=1, u=v=0.5 */
+  ORC_IC_SCALAR_PULSE = 10,  /* sin(2 pi x) + exp(-((x-0.5)/0.05)^2) */
+  ORC_IC_SCALAR_REGIONS = 11 /* params: default, sine(0/1), amplitude, k*3 (lower, upper, value) */
+};
+
+const char *orc_error(void);
+
+/* 2D Euler on straight-sided triangles. Nodes carry tags and (x, y);
+ * triangles list three node tags in file order (MSH records). boundary is
+ * ORC_PERIODIC (axis-aligned box, periodic donor traces) or ORC_OUTFLOW
+ * (transmissive: exterior = own trace, no limiter neighbour across). */
+orc_case *orc_tri_euler_create(int64_t node_count, const int64_t *node_tags, const double *node_xy,
+                               int64_t triangle_count, const int64_t *triangle_nodes, int order,
+                               double gamma, int boundary);
+
+/* 1D scalar advection on a line mesh. Segments list two node tags. */
+orc_case *orc_line_scalar_create(int64_t node_count, const int64_t *node_tags,
+                                 const double *node_x, int64_t segment_count,
+                                 const int64_t *segment_nodes, int order, double wavespeed,
+                                 int left_kind, double left_value, int right_kind,
+                                 double right_value);
+
+void orc_destroy(orc_case *c);
+
+/* Sizes: 0 elements, 1 faces, 2 modes, 3 variables, 4 edge points (p+1). */
+int64_t orc_size(const orc_case *c, int what);
+
+/* Canonical triangle topology (build_triangle_mesh.cpp:38-148).
+ * element_nodes [E][3] tags after CCW orientation; element_xy [E][6].
+ * face_elements [F][2] (left, right or -1); face_edges [F][2];
+ * face_vertices [F][4] (left v0, left v1, right v0, right v1);
+ * face_normals [F][2]. */
+int orc_tri_topology(const orc_case *c, int64_t *element_nodes, double *element_xy,
+                     int32_t *face_elements, int32_t *face_edges, int64_t *face_vertices,
+                     double *face_normals);
+
+/* Line topology (build_line_mesh.cpp:15-85): element_x [E][2] (left,right);
+ * face_elements [F][2]; face_normals [F]. */
+int orc_line_topology(const orc_case *c, double *element_x, int32_t *face_elements,
+                      double *face_normals);
+
+/* Periodic donor traces (periodic_triangle_trace.cpp:40-67):
+ * donors [F][p+1] (-1 on interior faces), basis [F][p+1][Np]. */
+int orc_tri_trace(const orc_case *c, int32_t *donors, double *basis);
+
+/* Limiter stencil (euler_limiter_stencil_*.cpp): neighbors [E][K],
+ * transforms [E][K][Np][Np], normals [E][2], check_basis [C][Np].
+ * K = 3 (triangles) or 2 (lines); C from orc_stencil_check_count. */
+int64_t orc_stencil_check_count(const orc_case *c);
+int orc_stencil(const orc_case *c, int32_t *neighbors, double *transforms, double *normals,
+                double *check_basis);
+
+/* One residual evaluation with the case's own boundary exterior
+ * (euler_triangle_residual / scalar_advection_residual). */
+int orc_residual(const orc_case *c, const double *state, double *rhs);
+
+/* Troubled-cell WENO + positivity limiter in place (euler_modal_limiter.cpp:70-98).
+ * flags [E] receives the sensor decision (1 troubled). */
+int orc_limit(const orc_case *c, double threshold, double density_floor, double pressure_floor,
+              double *state, uint8_t *flags);
+
+/* Stable time step (cfl_timestep.cpp). */
+int orc_stable_dt(const orc_case *c, const double *state, double cfl, double *dt);
+
+/* One SSP-RK3 step (ssp_rk3.hpp:37-61), limiter after every stage if enabled. */
+int orc_step(orc_case *c, double dt, int limiting, double threshold, double density_floor,
+             double pressure_floor, double *state);
+
+/* Density mass (Euler, euler_diagnostics.cpp:30-36) or scalar mass
+ * (scalar_diagnostics.cpp:18-25). */
+int orc_mass(const orc_case *c, const double *state, double *mass);
+
+/* L2 projection of an initial condition (initial_state.cpp:16-64). The case
+ * drivers limit the projected Euler state once when limiting is on
+ * (euler_triangle.cpp:37-41); callers do that with orc_limit. */
+int orc_project(const orc_case *c, int ic, const double *params, int64_t param_count,
+                double *state);
+
+/* The serial runner loop (serial_case_runner.cpp:117-149) starting from
+ * *state at start_time/start_step; writes the final state back and the
+ * history rows (step, time, dt, mass) at history_interval and the final step
+ * into history [rows][4] (capacity history_capacity rows). */
+int orc_run(orc_case *c, double cfl, int limiting, double threshold, double density_floor,
+            double pressure_floor, double start_time, uint64_t start_step, double end_time,
+            uint64_t max_steps, uint64_t history_interval, double *state, uint64_t *final_step,
+            double *final_time, double *history, int64_t history_capacity, int64_t *history_rows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
