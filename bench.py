@@ -1,0 +1,211 @@
+"""Benchmark of the RKDG hot path on one MI355X.
+
+Workload (the headline configuration): 2D compressible Euler, gamma 1.4,
+isentropic vortex (strength 5, centre (5,5), background rho=u=v=p=1) on
+periodic [0,10]^2, 2x128x128 = 32768 jittered triangles (+/-15% h, seed
+0x5EED), modal Dubiner P2 (6 modes x 4 variables), HLLC, no limiting, CFL
+0.15, FP64. A step is one full SSP-RK3 step of the whole mesh: three fused
+stage kernels with the CFL dt computed on the device.
+
+value = DOF-updates/s = E * Np * V * steps / wall time of the stepping loop
+(state resident in HBM; setup, projection and upload excluded).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Euler DG DOF-updates/s (P2, 32k tris) + p50/p99 solve latency, mixed vs plain"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def algorithmic_bytes_per_element_step(Np: int, V: int) -> float:
+    """Each array touched once per fused stage: stage 1 reads U^n and writes U1,
+    stages 2 and 3 read their input and U^n and write (8 S per step, S = Np V 8 B),
+    plus geometry 3 x (40 + 1.5 x 36) B and the CFL speed 32 B."""
+    S = Np * V * 8
+    return 8 * S + 3 * (40 + 1.5 * 36) + 32
+
+
+def flops_per_element_stage(p: int) -> float:
+    """FP64 operations of one stage per element (counted from the kernel):
+    modal->quadrature 2 Nq Np V, flux contraction 4 Nq Np V, pointwise flux
+    ~30 Nq, edges 3 (p+1) (2 traces 2 Np V + HLLC ~110 + lift 2 Np V) with
+    each face evaluated from both sides, plus 3 Np V for the RK combination."""
+    Np = (p + 1) * (p + 2) // 2
+    Nq = (p + 3) ** 2
+    V = 4
+    vol = 2 * Nq * Np * V + 4 * Nq * Np * V + 30 * Nq
+    face = 3 * (p + 1) * (2 * 2 * Np * V + 110 + 2 * Np * V)
+    return vol + face + 3 * Np * V
+
+
+def cpu_baseline(mesh, order, cfl, budget_s: float):
+    """The reference's own CPU path (oracle/_ref, compiled from the reference
+    sources) or, where that was not built, the project's C++ restatement,
+    timed single-threaded on a bounded number of steps of the same case."""
+    from oracle import oracle as O
+
+    kind = "reference" if O.available("reference") else "port"
+    c = O.tri_euler(mesh.node_tags, mesh.node_xy, mesh.triangles, order, kind=kind)
+    U = c.project(O.IC_VORTEX, (5.0, 5.0, 5.0))
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = c.stable_dt(U, cfl)
+        U = c.step(U, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 200:
+            break
+    dofs = c.E * c.Np * c.V
+    return {
+        "value": dofs * steps / el,
+        "unit": "DOF-updates/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"{steps} SSP-RK3 steps of the same P2 32768-triangle vortex case from t=0 "
+                  f"({el:.1f} s, single thread, {cpu_model()}, nproc {os.cpu_count()})",
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=128, help="quads per axis (2 n^2 triangles)")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine runs one case per GPU and the "
+              "multi-GPU path is not part of this build; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import wasl_amd
+    from wasl_amd import meshgen as MG
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+
+    p = args.order
+    mesh_in = MG.jittered_triangles(args.cells, (0.0, 0.0), (10.0, 10.0), 0.15)
+    mesh = wasl_amd.Mesh.from_input(mesh_in)
+    eng = wasl_amd.Engine(mesh, wasl_amd.EULER, p, lanes=args.lanes)
+    eng.project(wasl_amd.IC_VORTEX, (5.0, 5.0, 5.0))
+    E, Np, V = eng.E, eng.Np, eng.V
+    log(f"engine: {E} triangles, P{p}, lanes/element {eng.lanes}, "
+        f"{eng.device_bytes / 2**20:.1f} MiB on device")
+
+    cfl = 0.15
+    t, s, _ = eng.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    log(f"warmup {s} steps done (t = {t:.4f})")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, s, _ = eng.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                          stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = E * Np * V * args.steps / wall
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4e} DOF-updates/s")
+
+    # Stage-kernel duration: HIP events around every stage kernel on the
+    # engine's stream over a second pass of the same length.
+    eng.set_timing(True)
+    t, s, _ = eng.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                          stop_at_max_steps=True)
+    eng.set_timing(False)
+    _, stage_ms_total, launches = eng.last_timing()
+    stage_ms = stage_ms_total / max(launches, 1)
+    bytes_stage = E * algorithmic_bytes_per_element_step(Np, V) / 3.0
+    achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
+    flops_stage = E * flops_per_element_stage(p)
+    log(f"stage kernel: {stage_ms * 1e3:.2f} us average over {launches} launches, "
+        f"{achieved:.1f} GB/s algorithmic, {flops_stage / (stage_ms * 1e-3) / 1e12:.2f} TFLOP/s")
+
+    U = eng.state
+    finite = bool(np.isfinite(U).all())
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D Euler isentropic vortex, P{p} modal Dubiner, "
+                        f"2x{args.cells}x{args.cells}={E} jittered periodic triangles on [0,10]^2, "
+                        f"HLLC, SSP-RK3, CFL {cfl}, device dt, plain FP64",
+            "elements": E, "modes": Np, "variables": V,
+            "dofs": E * Np * V, "lanes_per_element": eng.lanes,
+            "stage_rate": 3 * args.steps / wall, "state_finite": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "whip::euler_stage",
+            "stage_kernel_us": stage_ms * 1e3,
+            "algorithmic_bytes_per_launch": bytes_stage,
+            "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,
+            "fp64_frac": flops_stage / (stage_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
+        result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.4e} DOF-updates/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

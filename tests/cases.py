@@ -1,0 +1,60 @@
+"""Matched cases for parity tests: the same raw mesh handed to the oracle and
+to the HIP engine, plus tolerance helpers.
+
+Tolerances (the parity contract of the plain FP64 path):
+  per residual / per step   relative L-inf <= 1e-12 (max-norm scaled)
+  end of run                relative L-inf <= 1e-10
+  connectivity, donors, stencil, troubled flags: bit-exact
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+from wasl_amd import meshgen as MG
+
+STEP_TOL = 1e-12
+RUN_TOL = 1e-10
+
+VORTEX = (5.0, 5.0, 5.0)
+RAREFACTION = (1.0, 0.4, 2.0)
+# rho = 1 + 0.1 sum_k a_k sin(2 pi (m_k x + n_k y) / L + phase_k), p = 1, u = v = 0.5
+SMOOTH = (2.0, 0.5, 1, 2, 0.3, -0.7, 2, 1, 1.1, 0.4, 3, 1, 2.0, -0.2, 1, 3, 0.5)
+# default region plus a low-density, low-pressure disc: troubled cells on its rim
+JUMP = (0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.2, 1.0,
+        2, 0, 0, 0.1, -0.2, 0.45, 0.2, 0.0, 0.0, 0.15)
+
+
+def rel_inf(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def rel_inf_per_variable(a, b, V: int) -> float:
+    """Max over variables of the max-norm-scaled difference (U[variable] blocks)."""
+    a = np.asarray(a).reshape(V, -1)
+    b = np.asarray(b).reshape(V, -1)
+    return max(rel_inf(a[v], b[v]) for v in range(V))
+
+
+def vortex_mesh(n=12, seed=MG.DEFAULT_SEED):
+    return MG.jittered_triangles(n, (0.0, 0.0), (10.0, 10.0), 0.15, seed)
+
+
+def rarefaction_mesh(n=10, seed=MG.DEFAULT_SEED):
+    return MG.jittered_triangles(n, (-1.0, -1.0), (1.0, 1.0), 0.15, seed)
+
+
+def oracle_tri(m, order, boundary=O.PERIODIC, kind="port"):
+    return O.tri_euler(m.node_tags, m.node_xy, m.triangles, order, boundary=boundary, kind=kind)
+
+
+def engine_tri(wasl, m, order, boundary=0, limiting=False, threshold=1e-3, floors=1e-10,
+               lanes=0, precision=0):
+    mesh = wasl.Mesh.from_input(m)
+    return wasl.Engine(mesh, wasl.EULER, order, boundary=(boundary, boundary), limiting=limiting,
+                       sensor_threshold=threshold, density_floor=floors, pressure_floor=floors,
+                       lanes=lanes, precision=precision)

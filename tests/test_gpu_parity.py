@@ -1,0 +1,210 @@
+"""GPU parity of the plain FP64 path against the CPU oracle.
+
+Every test runs the HIP kernels through the C ABI (wasl_amd/libwasl_hip.so)
+and compares with the oracle on the same seeded inputs:
+  residual / single step   relative L-inf <= 1e-12
+  multi-step runs          relative L-inf <= 1e-10
+  troubled-cell flags      bit-exact (a flag whose sensor ratio lies within
+                           1e-12 relative of the threshold is logged, not failed)
+"""
+
+import numpy as np
+import pytest
+
+import cases as K
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_euler_residual_periodic(gpu, order, lanes):
+    m = K.vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    g = K.engine_tri(gpu, m, order, lanes=lanes)
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    rng = np.random.default_rng(order)
+    U = U * (1.0 + 1e-3 * rng.standard_normal(U.size))
+    err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
+    print(f"P{order} lanes={lanes} residual rel L-inf {err:.3e}")
+    assert err <= K.STEP_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_euler_residual_transmissive(gpu, order):
+    m = K.rarefaction_mesh(10)
+    o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
+    g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW)
+    U = o.project(O.IC_SMOOTH, K.SMOOTH)
+    err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
+    print(f"P{order} transmissive residual rel L-inf {err:.3e}")
+    assert err <= K.STEP_TOL
+
+
+def test_residual_conserves_mass(gpu):
+    """Interior faces see identical flux bits from both sides: the density
+    residual integrates to ~0 on a periodic mesh."""
+    m = K.vortex_mesh(16)
+    g = K.engine_tri(gpu, m, 2)
+    mesh = gpu.Mesh.from_input(m)
+    U = mesh.project(2, gpu.IC_VORTEX, K.VORTEX)
+    R = g.residual(U)
+    detj = mesh.prepared()["element_metrics"][:, 4]
+    total = np.sum(np.sqrt(2.0) * detj * R[: g.E])
+    assert abs(total) <= 1e-12 * np.sum(np.abs(np.sqrt(2.0) * detj * R[: g.E]))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_stable_dt(gpu, order):
+    m = K.vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    g = K.engine_tri(gpu, m, order)
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    g.state = U
+    assert abs(g.stable_time_step(0.15) - o.stable_dt(U, 0.15)) <= 1e-15 * o.stable_dt(U, 0.15)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_step_unlimited(gpu, order):
+    m = K.vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    g = K.engine_tri(gpu, m, order)
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    dt = o.stable_dt(U, 0.15)
+    g.state = U
+    for k in range(3):
+        U = o.step(U, dt)
+        g.step(dt)
+        err = K.rel_inf_per_variable(g.state, U, 4)
+        print(f"P{order} step {k + 1} rel L-inf {err:.3e}")
+        assert err <= K.STEP_TOL
+
+
+def _sensor_ratio(U, E, Np, hb):
+    d = U[: Np * E].reshape(Np, E)
+    tot = np.sum(d * d, axis=0)
+    high = np.sum(d[hb:] ** 2, axis=0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return high / tot
+
+
+def _check_flags(gflags, oflags, U, E, Np, hb, threshold):
+    diff = np.nonzero(gflags != oflags)[0]
+    if diff.size:
+        ratio = _sensor_ratio(U, E, Np, hb)
+        near = np.abs(ratio[diff] - threshold) <= 1e-12 * threshold
+        for e in diff:
+            print(f"flag differs at element {e}: ratio {ratio[e]!r} threshold {threshold}")
+        assert near.all(), "troubled flags differ away from the threshold"
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_limiter_parity(gpu, order):
+    m = K.rarefaction_mesh(12)
+    o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
+    g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW, limiting=True, floors=1e-13)
+    U = o.project(O.IC_EULER_REGIONS, K.JUMP)
+    g.state = U
+    gflags = g.limit()
+    Uo, oflags = o.limit(U, 1e-3, 1e-13, 1e-13)
+    print(f"P{order}: {int(oflags.sum())} troubled of {g.E}")
+    assert oflags.sum() > 0
+    hb = (order) * (order + 1) // 2
+    _check_flags(gflags, oflags, U, g.E, g.Np, hb, 1e-3)
+    err = K.rel_inf_per_variable(g.state, Uo, 4)
+    print(f"P{order} limited state rel L-inf {err:.3e}")
+    assert err <= K.STEP_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_step_limited_periodic(gpu, order):
+    m = K.vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    g = K.engine_tri(gpu, m, order, limiting=True, threshold=1e-4)
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    rng = np.random.default_rng(3)
+    U = U * (1.0 + 0.01 * rng.standard_normal(U.size))
+    dt = o.stable_dt(U, 0.15)
+    g.state = U
+    for k in range(3):
+        U = o.step(U, dt, True, 1e-4)
+        g.step(dt)
+        err = K.rel_inf_per_variable(g.state, U, 4)
+        print(f"P{order} limited step {k + 1} rel L-inf {err:.3e}")
+        assert err <= K.STEP_TOL
+
+
+def test_advance_matches_oracle_run(gpu):
+    """Device-driven dt and end-time clipping against the runner loop."""
+    m = K.vortex_mesh(8)
+    o = K.oracle_tri(m, 2)
+    g = K.engine_tri(gpu, m, 2)
+    U0 = o.project(O.IC_VORTEX, K.VORTEX)
+    dt0 = o.stable_dt(U0, 0.15)
+    end = 20.5 * dt0
+    Uo, so, to, ho = o.run(U0, 0.15, end, 10 ** 6, 5)
+    g.state = U0
+    tg, sg, hg = g.advance(0.15, end, history_interval=5, history_capacity=64)
+    assert sg == so
+    assert abs(tg - to) <= 1e-13 * to
+    err = K.rel_inf_per_variable(g.state, Uo, 4)
+    print(f"advance {sg} steps rel L-inf {err:.3e}")
+    assert err <= K.RUN_TOL
+    assert hg.shape == ho.shape
+    np.testing.assert_array_equal(hg[:, 0], ho[:, 0])
+    np.testing.assert_allclose(hg[:, 1:3], ho[:, 1:3], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(hg[:, 3], ho[:, 3], rtol=1e-12, atol=0)
+
+
+def test_advance_max_steps_error(gpu):
+    m = K.vortex_mesh(6)
+    g = K.engine_tri(gpu, m, 1)
+    mesh = gpu.Mesh.from_input(m)
+    g.state = mesh.project(1, gpu.IC_VORTEX, K.VORTEX)
+    with pytest.raises(gpu.SimulationError, match="max_steps reached before end_time"):
+        g.advance(0.15, 100.0, max_steps=3)
+
+
+def test_unphysical_average_error(gpu):
+    m = K.vortex_mesh(6)
+    g = K.engine_tri(gpu, m, 1)
+    mesh = gpu.Mesh.from_input(m)
+    U = mesh.project(1, gpu.IC_VORTEX, K.VORTEX)
+    U[5] = -1.0  # negative density mean in element 5
+    g.state = U
+    with pytest.raises(gpu.SimulationError, match="Euler cell average is not physical"):
+        g.stable_time_step(0.15)
+    with pytest.raises(gpu.SimulationError, match="Euler cell average is not physical"):
+        g.advance(0.15, 1.0)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_line_advection_parity(gpu, order):
+    l = K.MG.jittered_line(200)
+    o = O.line_scalar(l.node_tags, l.node_x, l.segments, order, 1.0)
+    mesh = gpu.Mesh.from_input(l)
+    g = gpu.Engine(mesh, gpu.SCALAR_ADVECTION, order, wavespeed=1.0)
+    U = o.project(O.IC_SCALAR_PULSE)
+    assert K.rel_inf(g.residual(U), o.residual(U)) <= K.STEP_TOL
+    dt = o.stable_dt(U, 0.1)
+    assert g.stable_time_step(0.1) == dt
+    g.state = U
+    for _ in range(5):
+        U = o.step(U, dt)
+        g.step(dt)
+    assert K.rel_inf(g.state, U) <= K.STEP_TOL * 10
+
+
+@pytest.mark.parametrize("kinds", [(O.INFLOW, O.OUTFLOW), (O.OUTFLOW, O.INFLOW),
+                                   (O.OUTFLOW, O.OUTFLOW)])
+def test_line_boundaries(gpu, kinds):
+    l = K.MG.jittered_line(40)
+    a = -0.7 if kinds[1] == O.INFLOW else 0.7
+    o = O.line_scalar(l.node_tags, l.node_x, l.segments, 2, a, left=(kinds[0], 0.3),
+                      right=(kinds[1], -0.2))
+    mesh = gpu.Mesh.from_input(l)
+    g = gpu.Engine(mesh, gpu.SCALAR_ADVECTION, 2, wavespeed=a, boundary=kinds,
+                   boundary_values=(0.3, -0.2))
+    U = o.project(O.IC_SCALAR_PULSE)
+    assert K.rel_inf(g.residual(U), o.residual(U)) <= K.STEP_TOL

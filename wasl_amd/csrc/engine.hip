@@ -1,0 +1,1153 @@
+// wasl_hip engine: device memory, kernel dispatch and the C ABI of
+// include/wasl_hip.h. Host setup lives in setup.cpp, device code in
+// kernels.hpp.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <unordered_map>
+#include <string>
+#include <vector>
+
+#include "../../include/wasl_hip.h"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+struct wasl_hip_mesh {
+  whip::Mesh m;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct SimulationError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw DeviceError(std::string("hip: ") + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) check((x), #x)
+
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return WASL_HIP_OK;
+  } catch (const SimulationError &ex) {
+    g_error = ex.what();
+    return WASL_HIP_SIMULATION;
+  } catch (const DeviceError &ex) {
+    g_error = ex.what();
+    return WASL_HIP_DEVICE;
+  } catch (const std::exception &ex) {
+    g_error = ex.what();
+    return WASL_HIP_ERROR;
+  }
+}
+
+const char *device_error_text(int code) {
+  switch (code) {
+  case whip::kErrUnphysical:
+    return "simulation: Euler cell average is not physical";
+  case whip::kErrPositivity:
+    return "Euler positivity limiter: cell average is below a floor";
+  case whip::kErrTimestep:
+    return "simulation: timestep cannot advance time";
+  default:
+    return "simulation: device error";
+  }
+}
+
+// Device allocation owned by the engine.
+struct Buffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+  Buffer() = default;
+  Buffer(const Buffer &) = delete;
+  Buffer &operator=(const Buffer &) = delete;
+  ~Buffer() {
+    if (p)
+      (void)hipFree(p);
+  }
+  void alloc(size_t n, size_t &total) {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    bytes = n;
+    if (n) {
+      HIPCHECK(hipMalloc(&p, n));
+      total += n;
+    }
+  }
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <typename T>
+void upload(Buffer &b, const std::vector<T> &v, size_t &total, hipStream_t s) {
+  b.alloc(v.size() * sizeof(T), total);
+  if (!v.empty())
+    HIPCHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+int choose_lanes(int64_t elements) {
+  if (elements < 131072)
+    return 4;
+  if (elements < 524288)
+    return 2;
+  return 1;
+}
+
+}  // namespace
+
+struct wasl_hip_engine {
+  wasl_hip_numerics num{};
+  int dim = 2, p = 0, Np = 1, V = 4;
+  int E = 0, F = 0, NB = 0;
+  int lanes = 1;
+  double hmin = 0.0;
+  hipStream_t stream = nullptr;
+  size_t device_bytes = 0;
+
+  whip::PeriodicTrace trace;      // host copy for export
+  whip::LimiterStencil stencil;   // host copy (built when limiting or on request)
+  bool have_stencil = false;
+  whip::Mesh mesh;                // kept for stencil export and projection
+
+  // state and stage buffers
+  Buffer u, w1, w2;
+  // geometry and tables
+  Buffer geo, nbr, meta, fn, bdonor, bbasis, table, inv_jac;
+  // limiter
+  Buffer lnbr, T, nrm, check, flags, list, counters, side, err;
+  // control and reductions
+  Buffer ctl, mass_w, partials, result;
+  whip::RunCtl *ctl_host = nullptr;   // pinned
+  double *result_host = nullptr;      // pinned
+
+  // timing of the last step/advance call
+  std::vector<hipEvent_t> ev;
+  double last_wall_ms = 0.0, last_stage_ms = 0.0;
+  int64_t last_stage_launches = 0;
+  bool timing = false;
+
+  ~wasl_hip_engine() {
+    for (auto e : ev)
+      (void)hipEventDestroy(e);
+    if (ctl_host)
+      (void)hipHostFree(ctl_host);
+    if (result_host)
+      (void)hipHostFree(result_host);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+
+  size_t state_size() const { return static_cast<size_t>(V) * Np * E; }
+
+  whip::EulerMesh euler_mesh() const {
+    whip::EulerMesh m;
+    m.E = E;
+    m.geo = geo.as<double>();
+    m.nbr = nbr.as<int>();
+    m.meta = meta.as<int>();
+    m.fn = fn.as<double>();
+    m.bdonor = bdonor.as<int>();
+    m.bbasis = bbasis.as<double>();
+    m.table = table.as<double>();
+    m.gamma = num.heat_capacity_ratio;
+    return m;
+  }
+
+  whip::LineMeshDev line_mesh() const {
+    whip::LineMeshDev m;
+    m.E = E;
+    m.inv_jac = inv_jac.as<double>();
+    m.table = table.as<double>();
+    m.wavespeed = num.wavespeed;
+    m.left_kind = num.boundary_left;
+    m.right_kind = num.boundary_right;
+    m.left_value = num.boundary_value_left;
+    m.right_value = num.boundary_value_right;
+    return m;
+  }
+
+  whip::LimiterDev limiter() const {
+    whip::LimiterDev L;
+    L.E = E;
+    L.nbr = lnbr.as<int>();
+    L.T = T.as<double>();
+    L.nrm = nrm.as<double>();
+    L.check = check.as<double>();
+    L.gamma = num.heat_capacity_ratio;
+    L.density_floor = num.density_floor;
+    L.pressure_floor = num.pressure_floor;
+    return L;
+  }
+
+  void build_stencil_host() {
+    if (have_stencil || dim != 2)
+      return;
+    stencil = whip::limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC
+                                                 ? trace.donors
+                                                 : std::vector<int32_t>{},
+                                    p + 1);
+    have_stencil = true;
+  }
+};
+
+namespace {
+
+// ------------------------------------------------------------- tables ----
+
+std::vector<double> euler_tables(int p) {
+  const int Np = whip::tri_modes(p), NQ = (p + 3) * (p + 3), NE = p + 1;
+  const int kWx = NQ * Np, kWy = 2 * NQ * Np, kBl = 3 * NQ * Np;
+  const int kBr = kBl + 3 * NE * Np, kWe = kBr + 18 * NE * Np;
+  std::vector<double> t(kWe + 3 * NE, 0.0);
+  const whip::TriRule vol = whip::triangle_rule(p + 2);
+  std::vector<double> b(Np), gx(Np), ge(Np);
+  for (int q = 0; q < NQ; ++q) {
+    whip::dubiner(vol.xi[q], vol.eta[q], p, b.data());
+    whip::dubiner_gradient(vol.xi[q], vol.eta[q], p, gx.data(), ge.data());
+    for (int m = 0; m < Np; ++m) {
+      t[q * Np + m] = b[m];
+      t[kWx + q * Np + m] = vol.w[q] * gx[m];
+      t[kWy + q * Np + m] = vol.w[q] * ge[m];
+    }
+  }
+  const auto edges = whip::edge_rules(p);
+  for (int k = 0; k < 3; ++k)
+    for (int q = 0; q < NE; ++q) {
+      whip::dubiner(edges[k].xi[q], edges[k].eta[q], p, t.data() + kBl + (k * NE + q) * Np);
+      t[kWe + k * NE + q] = edges[k].w[q];
+    }
+  for (int kl = 0; kl < 3; ++kl)
+    for (int kr = 0; kr < 3; ++kr)
+      for (int fl = 0; fl < 2; ++fl)
+        for (int q = 0; q < NE; ++q) {
+          double rxi, reta;
+          whip::neighbor_point(kl, kr, fl != 0, edges[kl].xi[q], edges[kl].eta[q], rxi, reta);
+          whip::dubiner(rxi, reta, p, t.data() + kBr + (((kl * 3 + kr) * 2 + fl) * NE + q) * Np);
+        }
+  return t;
+}
+
+std::vector<double> line_tables(int p) {
+  const int Np = p + 1, NQ = p > 0 ? p : 1;
+  std::vector<double> t(2 * NQ * Np + 2 * Np, 0.0);
+  const whip::LineRule g = whip::gauss_legendre(NQ);
+  std::vector<double> b(Np), d(Np);
+  for (int q = 0; q < NQ; ++q) {
+    whip::legendre(g.x[q], p, b.data());
+    whip::legendre_gradient(g.x[q], p, d.data());
+    for (int m = 0; m < Np; ++m) {
+      t[q * Np + m] = b[m];
+      t[NQ * Np + q * Np + m] = g.w[q] * d[m];
+    }
+  }
+  whip::legendre(-1.0, p, t.data() + 2 * NQ * Np);
+  whip::legendre(1.0, p, t.data() + 2 * NQ * Np + Np);
+  return t;
+}
+
+// ------------------------------------------------------------- launch ----
+
+template <int P>
+void launch_euler_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
+  const whip::EulerMesh m = g.euler_mesh();
+  const int per = 256 / g.lanes;
+  const dim3 grid((g.E + per - 1) / per);
+  switch (g.lanes) {
+  case 1:
+    hipLaunchKernelGGL((whip::euler_stage<P, 1>), grid, dim3(256), 0, g.stream, m, a);
+    break;
+  case 2:
+    hipLaunchKernelGGL((whip::euler_stage<P, 2>), grid, dim3(256), 0, g.stream, m, a);
+    break;
+  default:
+    hipLaunchKernelGGL((whip::euler_stage<P, 4>), grid, dim3(256), 0, g.stream, m, a);
+    break;
+  }
+}
+
+template <int P> void launch_line_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
+  const dim3 grid((g.E + 255) / 256);
+  hipLaunchKernelGGL((whip::line_stage<P>), grid, dim3(256), 0, g.stream, g.line_mesh(), a);
+}
+
+void launch_stage(const wasl_hip_engine &g, const whip::StageArgs &a) {
+  if (g.dim == 2) {
+    switch (g.p) {
+    case 0: launch_euler_p<0>(g, a); break;
+    case 1: launch_euler_p<1>(g, a); break;
+    case 2: launch_euler_p<2>(g, a); break;
+    default: launch_euler_p<3>(g, a); break;
+    }
+  } else {
+    switch (g.p) {
+    case 0: launch_line_p<0>(g, a); break;
+    case 1: launch_line_p<1>(g, a); break;
+    case 2: launch_line_p<2>(g, a); break;
+    default: launch_line_p<3>(g, a); break;
+    }
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+template <int P>
+void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip::RunCtl *ctl,
+                    int parity) {
+  const whip::LimiterDev L = g.limiter();
+  const dim3 grid((g.E + 63) / 64), grid256((g.E + 255) / 256);
+  hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, g.stream, L, U,
+                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.err.as<int>());
+  hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, g.stream, U, g.E,
+                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity);
+}
+
+// Limit the troubled elements listed by stage `slot` (counter index).
+void launch_limit(const wasl_hip_engine &g, double *U, int slot, whip::RunCtl *ctl, int parity) {
+  const int *count = g.counters.as<int>() + slot;
+  switch (g.p) {
+  case 0: return;  // P0 is returned unchanged (euler_modal_limiter.cpp:73-74)
+  case 1: launch_limit_p<1>(g, U, count, ctl, parity); break;
+  case 2: launch_limit_p<2>(g, U, count, ctl, parity); break;
+  default: launch_limit_p<3>(g, U, count, ctl, parity); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+template <int P> void launch_sensor_p(const wasl_hip_engine &g, uint8_t *flags) {
+  hipLaunchKernelGGL((whip::sensor<P>), dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
+                     g.u.as<double>(), g.E, g.num.sensor_threshold, g.stencil.highest_begin,
+                     flags, g.list.as<int>(), g.counters.as<int>() + 3);
+}
+
+bool limiting(const wasl_hip_engine &g) { return g.dim == 2 && g.num.limiting && g.p > 0; }
+
+// Shu-Osher stages (time/ssp_rk3.hpp:45-60): U1 = U + dt R(U);
+// U2 = 3/4 U + 1/4 (U1 + dt R(U1)); U = 1/3 U + 2/3 (U2 + dt R(U2)); the
+// limiter after each.
+void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, bool timed,
+                 int &timed_index) {
+  double *U = g.u.as<double>(), *A = g.w1.as<double>(), *B = g.w2.as<double>();
+  const double *in[3] = {U, A, B};
+  double *out[3] = {A, B, U};
+  const double ca[3] = {0.0, 0.75, 1.0 / 3.0};
+  const double cb[3] = {1.0, 0.25, 2.0 / 3.0};
+  const bool lim = limiting(g);
+  for (int s = 0; s < 3; ++s) {
+    whip::StageArgs a{};
+    a.uin = in[s];
+    a.un = U;
+    a.out = out[s];
+    a.ca = ca[s];
+    a.cb = cb[s];
+    a.dt = dt;
+    a.ctl = ctl;
+    a.parity = parity;
+    a.stage = s;
+    a.residual_only = 0;
+    a.want_lambda = (g.dim == 2 && ctl && s == 2) ? 1 : 0;
+    a.sensor = lim ? 1 : 0;
+    a.threshold = g.num.sensor_threshold;
+    a.highest_begin = g.stencil.highest_begin;
+    a.flags = nullptr;
+    a.list = g.list.as<int>();
+    a.counters = g.counters.as<int>();
+    if (timed)
+      HIPCHECK(hipEventRecord(g.ev[2 * timed_index], g.stream));
+    launch_stage(g, a);
+    if (timed) {
+      HIPCHECK(hipEventRecord(g.ev[2 * timed_index + 1], g.stream));
+      ++timed_index;
+    }
+    if (lim)
+      launch_limit(g, out[s], s, ctl, parity);
+  }
+}
+
+void ensure_events(wasl_hip_engine &g, size_t pairs) {
+  while (g.ev.size() < 2 * pairs) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    g.ev.push_back(e);
+  }
+}
+
+// Sum of the per-stage event intervals recorded by launch_step.
+double stage_time_ms(wasl_hip_engine &g, int pairs) {
+  double total = 0.0;
+  for (int i = 0; i < pairs; ++i) {
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, g.ev[2 * i], g.ev[2 * i + 1]));
+    total += ms;
+  }
+  return total;
+}
+
+void raise_device_error(wasl_hip_engine &g) {
+  int code = 0;
+  HIPCHECK(hipMemcpyAsync(&code, g.err.p, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  if (code != 0) {
+    HIPCHECK(hipMemsetAsync(g.err.p, 0, sizeof(int), g.stream));
+    HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
+    throw SimulationError(device_error_text(code));
+  }
+}
+
+double mass_of(wasl_hip_engine &g) {
+  const int blocks = (g.E + 255) / 256;
+  hipLaunchKernelGGL(whip::mass_partials, dim3(blocks), dim3(256), 0, g.stream,
+                     g.u.as<double>(), g.dim == 2 ? 1 : g.Np, g.mass_w.as<double>(), g.E,
+                     g.partials.as<double>());
+  hipLaunchKernelGGL(whip::mass_finish, dim3(1), dim3(64), 0, g.stream, g.partials.as<double>(),
+                     blocks, g.result.as<double>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(g.result_host, g.result.p, sizeof(double), hipMemcpyDeviceToHost,
+                          g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  return *g.result_host;
+}
+
+// Largest |u|+c of the resident state's cell averages (throws when one is
+// unphysical), or |a| for advection.
+double resident_speed(wasl_hip_engine &g) {
+  if (g.dim == 1)
+    return std::fabs(g.num.wavespeed);
+  whip::RunCtl *h = g.ctl_host;
+  std::memset(h, 0, sizeof(*h));
+  HIPCHECK(hipMemcpyAsync(g.ctl.p, h, sizeof(*h), hipMemcpyHostToDevice, g.stream));
+  hipLaunchKernelGGL(whip::cfl_speed, dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
+                     g.u.as<double>(), g.E, g.Np, g.num.heat_capacity_ratio,
+                     g.ctl.as<whip::RunCtl>(), 0);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(h, g.ctl.p, sizeof(*h), hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  if (h->s[0].bad)
+    throw SimulationError("simulation: Euler cell average is not physical");
+  double speed;
+  std::memcpy(&speed, &h->s[0].lambda, sizeof(double));
+  return speed;
+}
+
+void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hip_engine &g) {
+  g.num = num;
+  g.p = num.polynomial_order;
+  if (g.p < 0 || g.p > 3)
+    throw std::runtime_error("engine: polynomial order must be 0..3");
+  if (num.precision != WASL_HIP_FP64)
+    throw std::runtime_error("engine: only the FP64 precision is available in this build");
+  g.mesh = mesh;
+  HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.ctl_host), sizeof(whip::RunCtl)));
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.result_host), sizeof(double)));
+  size_t &tot = g.device_bytes;
+
+  if (num.equation == WASL_HIP_EULER) {
+    if (mesh.dimension != 2)
+      throw std::runtime_error("engine: Euler runs on triangle meshes");
+    if (num.boundary_left != WASL_HIP_PERIODIC && num.boundary_left != WASL_HIP_OUTFLOW)
+      throw std::runtime_error("simulation: 2D Euler boundaries must be periodic or outflow");
+    g.dim = 2;
+    g.V = 4;
+    g.Np = whip::tri_modes(g.p);
+    g.E = static_cast<int>(mesh.tris.size());
+    g.F = static_cast<int>(mesh.faces.size());
+    g.lanes = num.lanes_per_element > 0 ? num.lanes_per_element : choose_lanes(g.E);
+    if (g.lanes != 1 && g.lanes != 2 && g.lanes != 4)
+      throw std::runtime_error("engine: lanes_per_element must be 1, 2 or 4");
+    const int NE = g.p + 1;
+    const size_t E = static_cast<size_t>(g.E);
+
+    // element metrics
+    std::vector<double> geo(5 * E), mw(E);
+    for (size_t e = 0; e < E; ++e) {
+      const whip::Triangle &t = mesh.tris[e];
+      geo[e] = 0.5 * t.by();
+      geo[E + e] = -0.5 * t.bx();
+      geo[2 * E + e] = -0.5 * t.ay();
+      geo[3 * E + e] = 0.5 * t.ax();
+      geo[4 * E + e] = 1.0 / t.jacobian_determinant();
+      mw[e] = std::sqrt(2.0) * t.jacobian_determinant();
+    }
+    // edge records
+    if (num.boundary_left == WASL_HIP_PERIODIC)
+      g.trace = whip::periodic_trace(mesh, g.p);
+    std::vector<int> nbr(3 * E, -2), meta(3 * E, 0);
+    std::vector<double> fn(9 * E, 0.0);
+    std::vector<int> bdon;
+    std::vector<double> bbas;
+    int slot = 0;
+    for (size_t f = 0; f < mesh.faces.size(); ++f) {
+      const whip::TriFace &x = mesh.faces[f];
+      const whip::Triangle &l = mesh.tris[x.left];
+      const double scale = l.edge_length(x.left_edge) / whip::reference_edge_length(x.left_edge);
+      const int flip = x.boundary() ? 0 : (x.flip() ? 1 : 0);
+      const int kr = x.boundary() ? 0 : x.right_edge;
+      int m = (x.left_edge << 1) | (kr << 3) | (flip << 5);
+      const auto put = [&](int e, int k, int other, int bits) {
+        const size_t i = static_cast<size_t>(k) * E + e;
+        if (nbr[i] != -2)
+          throw std::runtime_error("engine: element edge belongs to two faces");
+        nbr[i] = other;
+        meta[i] = bits;
+        fn[(3 * k) * E + e] = x.nx;
+        fn[(3 * k + 1) * E + e] = x.ny;
+        fn[(3 * k + 2) * E + e] = scale;
+      };
+      if (x.boundary()) {
+        put(x.left, x.left_edge, -1, m | whip::kMetaLeft | whip::kMetaBoundary | (slot << 8));
+        if (num.boundary_left == WASL_HIP_PERIODIC) {
+          for (int q = 0; q < NE; ++q) {
+            bdon.push_back(g.trace.donors[f * NE + q]);
+            for (int k = 0; k < g.Np; ++k)
+              bbas.push_back(g.trace.basis[(f * NE + q) * g.Np + k]);
+          }
+        }
+        ++slot;
+      } else {
+        put(x.left, x.left_edge, x.right, m | whip::kMetaLeft);
+        put(x.right, x.right_edge, x.left, m);
+      }
+    }
+    if (std::find(nbr.begin(), nbr.end(), -2) != nbr.end())
+      throw std::runtime_error("engine: element edge without a face");
+    g.NB = slot;
+    upload(g.geo, geo, tot, g.stream);
+    upload(g.mass_w, mw, tot, g.stream);
+    upload(g.nbr, nbr, tot, g.stream);
+    upload(g.meta, meta, tot, g.stream);
+    upload(g.fn, fn, tot, g.stream);
+    upload(g.bdonor, bdon, tot, g.stream);
+    upload(g.bbasis, bbas, tot, g.stream);
+    upload(g.table, euler_tables(g.p), tot, g.stream);
+    g.hmin = whip::minimum_length(mesh);
+
+    if (num.limiting && g.p > 0) {
+      g.build_stencil_host();
+      upload(g.lnbr, g.stencil.neighbors, tot, g.stream);
+      upload(g.T, g.stencil.transforms, tot, g.stream);
+      std::vector<double> nrm(2 * E);
+      for (size_t e = 0; e < E; ++e) {
+        nrm[2 * e] = g.stencil.nx[e];
+        nrm[2 * e + 1] = g.stencil.ny[e];
+      }
+      upload(g.nrm, nrm, tot, g.stream);
+      upload(g.check, g.stencil.check, tot, g.stream);
+      g.side.alloc(E * 4 * g.Np * sizeof(double), tot);
+    } else {
+      g.stencil.highest_begin = g.p > 0 ? whip::tri_modes(g.p - 1) : g.Np;
+    }
+    g.flags.alloc(E, tot);
+    g.list.alloc(E * sizeof(int), tot);
+  } else if (num.equation == WASL_HIP_SCALAR_ADVECTION) {
+    if (mesh.dimension != 1)
+      throw std::runtime_error("engine: scalar advection runs on line meshes");
+    if (num.limiting)
+      throw std::runtime_error("check: scalar limiting is not supported");
+    const int kinds[2] = {num.boundary_left, num.boundary_right};
+    for (int k : kinds)
+      if (k < 0 || k > 2)
+        throw std::runtime_error("boundaries: invalid boundary kind");
+    if ((num.boundary_left == WASL_HIP_PERIODIC) != (num.boundary_right == WASL_HIP_PERIODIC))
+      throw std::runtime_error("boundaries: periodic ends must be paired");
+    g.dim = 1;
+    g.V = 1;
+    g.Np = g.p + 1;
+    g.E = static_cast<int>(mesh.lines.size());
+    g.F = static_cast<int>(mesh.lfaces.size());
+    g.lanes = 1;
+    std::vector<double> ij(g.E), mw(g.E);
+    for (int e = 0; e < g.E; ++e) {
+      const double jac = (mesh.lines[e].xr - mesh.lines[e].xl) / 2.0;
+      ij[e] = 1.0 / jac;
+      mw[e] = std::sqrt(2.0) * jac;
+    }
+    upload(g.inv_jac, ij, tot, g.stream);
+    upload(g.mass_w, mw, tot, g.stream);
+    upload(g.table, line_tables(g.p), tot, g.stream);
+    g.hmin = whip::minimum_length(mesh);
+  } else {
+    throw std::runtime_error("engine: unknown equation");
+  }
+  const size_t n = g.state_size() * sizeof(double);
+  g.u.alloc(n, tot);
+  g.w1.alloc(n, tot);
+  g.w2.alloc(n, tot);
+  HIPCHECK(hipMemsetAsync(g.u.p, 0, n, g.stream));
+  g.counters.alloc(4 * sizeof(int), tot);
+  HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
+  g.err.alloc(sizeof(int), tot);
+  HIPCHECK(hipMemsetAsync(g.err.p, 0, sizeof(int), g.stream));
+  g.ctl.alloc(sizeof(whip::RunCtl), tot);
+  g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
+  g.result.alloc(sizeof(double), tot);
+  HIPCHECK(hipStreamSynchronize(g.stream));
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char *wasl_hip_last_error(void) { return g_error.c_str(); }
+
+int wasl_hip_mesh_build(int dimension, int64_t node_count, const int64_t *node_tags,
+                        const double *node_xyz, int64_t element_count,
+                        const int64_t *element_nodes, wasl_hip_mesh **out) {
+  return guarded([&] {
+    if (!out || !node_tags || !node_xyz || !element_nodes)
+      throw std::runtime_error("mesh: null argument");
+    auto m = std::make_unique<wasl_hip_mesh>();
+    m->m = whip::build_mesh(dimension, node_count, node_tags, node_xyz, element_count,
+                            element_nodes);
+    *out = m.release();
+  });
+}
+
+int wasl_hip_mesh_from_prepared(int dimension, int64_t node_count, const int64_t *node_tags,
+                                const double *points, int64_t element_count,
+                                const int64_t *element_nodes, int64_t face_count,
+                                const int64_t *face_nodes, const int32_t *face_elements,
+                                const int32_t *face_local_edges, const double *face_normals,
+                                wasl_hip_mesh **out) {
+  return guarded([&] {
+    if (!out)
+      throw std::runtime_error("mesh: null argument");
+    auto m = std::make_unique<wasl_hip_mesh>();
+    m->m = whip::mesh_from_prepared(dimension, node_count, node_tags, points, element_count,
+                                    element_nodes, face_count, face_nodes, face_elements,
+                                    face_local_edges, face_normals);
+    *out = m.release();
+  });
+}
+
+int64_t wasl_hip_mesh_count(const wasl_hip_mesh *mesh, int what) {
+  if (!mesh)
+    return -1;
+  const whip::Mesh &m = mesh->m;
+  switch (what) {
+  case 0:
+    return static_cast<int64_t>(m.elements());
+  case 1:
+    return static_cast<int64_t>(m.face_count());
+  case 2:
+    return static_cast<int64_t>(m.boundary_faces());
+  case 3: {
+    std::vector<int64_t> tags;
+    if (m.dimension == 2)
+      for (const auto &t : m.tris)
+        tags.insert(tags.end(), t.node, t.node + 3);
+    else
+      for (const auto &l : m.lines) {
+        tags.push_back(l.left_node);
+        tags.push_back(l.right_node);
+      }
+    std::sort(tags.begin(), tags.end());
+    return static_cast<int64_t>(std::unique(tags.begin(), tags.end()) - tags.begin());
+  }
+  default:
+    return -1;
+  }
+}
+
+// prepare_mesh (src/mesh/prepared/prepare_mesh.cpp:79-140): nodes numbered in
+// order of first use by the elements.
+int wasl_hip_mesh_export(const wasl_hip_mesh *mesh, int64_t *node_tags, double *points,
+                         int64_t *element_nodes, double *element_metrics, int64_t *face_nodes,
+                         int32_t *face_elements, int32_t *face_local_edges, double *face_normals,
+                         double *face_measures) {
+  return guarded([&] {
+    if (!mesh)
+      throw std::runtime_error("mesh: null argument");
+    const whip::Mesh &m = mesh->m;
+    std::unordered_map<int64_t, int64_t> local;
+    std::vector<int64_t> tags;
+    std::vector<double> pts;
+    const auto add = [&](int64_t tag, double x, double y) {
+      auto it = local.find(tag);
+      if (it != local.end())
+        return it->second;
+      const int64_t i = static_cast<int64_t>(tags.size());
+      local.emplace(tag, i);
+      tags.push_back(tag);
+      pts.insert(pts.end(), {x, y, 0.0});
+      return i;
+    };
+    if (m.dimension == 2) {
+      for (size_t e = 0; e < m.tris.size(); ++e) {
+        const whip::Triangle &t = m.tris[e];
+        for (int k = 0; k < 3; ++k) {
+          const int64_t i = add(t.node[k], t.x[k], t.y[k]);
+          if (element_nodes)
+            element_nodes[3 * e + k] = i;
+        }
+        if (element_metrics) {
+          const double v[5] = {t.ax() / 2.0, t.bx() / 2.0, t.ay() / 2.0, t.by() / 2.0,
+                               t.jacobian_determinant()};
+          std::copy(v, v + 5, element_metrics + 5 * e);
+        }
+      }
+      for (size_t f = 0; f < m.faces.size(); ++f) {
+        const whip::TriFace &x = m.faces[f];
+        if (face_nodes) {
+          face_nodes[2 * f] = local.at(x.va);
+          face_nodes[2 * f + 1] = local.at(x.vb);
+        }
+        if (face_elements) {
+          face_elements[2 * f] = x.left;
+          face_elements[2 * f + 1] = x.right;
+        }
+        if (face_local_edges) {
+          face_local_edges[2 * f] = x.left_edge;
+          face_local_edges[2 * f + 1] = x.right_edge;
+        }
+        if (face_normals) {
+          face_normals[2 * f] = x.nx;
+          face_normals[2 * f + 1] = x.ny;
+        }
+        if (face_measures)
+          face_measures[f] = m.tris[x.left].edge_length(x.left_edge);
+      }
+    } else {
+      for (size_t e = 0; e < m.lines.size(); ++e) {
+        const whip::Segment &s = m.lines[e];
+        const int64_t a = add(s.left_node, s.xl, 0.0), b = add(s.right_node, s.xr, 0.0);
+        if (element_nodes) {
+          element_nodes[2 * e] = a;
+          element_nodes[2 * e + 1] = b;
+        }
+        if (element_metrics)
+          element_metrics[e] = (s.xr - s.xl) / 2.0;
+      }
+      for (size_t f = 0; f < m.lfaces.size(); ++f) {
+        const whip::LineFace &x = m.lfaces[f];
+        const whip::Segment &s = m.lines[x.left];
+        const int64_t node = local.at(x.x == s.xl ? s.left_node : s.right_node);
+        if (face_nodes) {
+          face_nodes[2 * f] = node;
+          face_nodes[2 * f + 1] = node;
+        }
+        if (face_elements) {
+          face_elements[2 * f] = x.left;
+          face_elements[2 * f + 1] = x.right;
+        }
+        if (face_local_edges) {
+          face_local_edges[2 * f] = x.normal < 0.0 ? 0 : 1;
+          face_local_edges[2 * f + 1] = 0;
+        }
+        if (face_normals)
+          face_normals[f] = x.normal;
+        if (face_measures)
+          face_measures[f] = 1.0;
+      }
+    }
+    if (node_tags)
+      std::copy(tags.begin(), tags.end(), node_tags);
+    if (points)
+      std::copy(pts.begin(), pts.end(), points);
+  });
+}
+
+int wasl_hip_mesh_face_vertices(const wasl_hip_mesh *mesh, int64_t *vertices) {
+  return guarded([&] {
+    if (!mesh || mesh->m.dimension != 2)
+      throw std::runtime_error("mesh: face vertices exist for triangle meshes");
+    for (size_t f = 0; f < mesh->m.faces.size(); ++f) {
+      const whip::TriFace &x = mesh->m.faces[f];
+      vertices[4 * f] = x.lv0;
+      vertices[4 * f + 1] = x.lv1;
+      vertices[4 * f + 2] = x.rv0;
+      vertices[4 * f + 3] = x.rv1;
+    }
+  });
+}
+
+void wasl_hip_mesh_destroy(wasl_hip_mesh *mesh) { delete mesh; }
+
+int wasl_hip_mesh_periodic_trace(const wasl_hip_mesh *mesh, int p, int32_t *donors,
+                                 double *basis) {
+  return guarded([&] {
+    if (!mesh || mesh->m.dimension != 2 || p < 0 || p > 3)
+      throw std::runtime_error("trace: a triangle mesh and order 0..3 are required");
+    const whip::PeriodicTrace t = whip::periodic_trace(mesh->m, p);
+    std::copy(t.donors.begin(), t.donors.end(), donors);
+    std::copy(t.basis.begin(), t.basis.end(), basis);
+  });
+}
+
+int wasl_hip_mesh_limiter_stencil(const wasl_hip_mesh *mesh, int p, int periodic,
+                                  int32_t *neighbors, double *transforms, double *normals,
+                                  double *check_basis) {
+  return guarded([&] {
+    if (!mesh || mesh->m.dimension != 2 || p < 0 || p > 3)
+      throw std::runtime_error("stencil: a triangle mesh and order 0..3 are required");
+    std::vector<int32_t> donors;
+    if (periodic)
+      donors = whip::periodic_trace(mesh->m, p).donors;
+    const whip::LimiterStencil s = whip::limiter_stencil(mesh->m, p, donors, p + 1);
+    if (neighbors)
+      std::copy(s.neighbors.begin(), s.neighbors.end(), neighbors);
+    if (transforms)
+      std::copy(s.transforms.begin(), s.transforms.end(), transforms);
+    if (normals)
+      for (size_t e = 0; e < s.nx.size(); ++e) {
+        normals[2 * e] = s.nx[e];
+        normals[2 * e + 1] = s.ny[e];
+      }
+    if (check_basis)
+      std::copy(s.check.begin(), s.check.end(), check_basis);
+  });
+}
+
+int wasl_hip_mesh_project(const wasl_hip_mesh *mesh, int p, double gamma, int ic,
+                          const double *params, int64_t param_count, double *state) {
+  return guarded([&] {
+    if (!mesh || !state || p < 0 || p > 3)
+      throw std::runtime_error("project: invalid argument");
+    const std::vector<double> U = whip::project(mesh->m, p, gamma, ic, params, param_count);
+    std::copy(U.begin(), U.end(), state);
+  });
+}
+
+void wasl_hip_numerics_default(wasl_hip_numerics *n) {
+  if (!n)
+    return;
+  std::memset(n, 0, sizeof(*n));
+  n->equation = WASL_HIP_EULER;
+  n->polynomial_order = 1;
+  n->heat_capacity_ratio = 1.4;
+  n->sensor_threshold = 1e-3;
+  n->density_floor = 1e-10;
+  n->pressure_floor = 1e-10;
+  n->precision = WASL_HIP_FP64;
+}
+
+int wasl_hip_engine_create(const wasl_hip_mesh *mesh, const wasl_hip_numerics *numerics,
+                           wasl_hip_engine **out) {
+  return guarded([&] {
+    if (!mesh || !numerics || !out)
+      throw std::runtime_error("engine: null argument");
+    auto g = std::make_unique<wasl_hip_engine>();
+    create_engine(mesh->m, *numerics, *g);
+    *out = g.release();
+  });
+}
+
+void wasl_hip_engine_destroy(wasl_hip_engine *engine) { delete engine; }
+
+int64_t wasl_hip_engine_size(const wasl_hip_engine *g, int what) {
+  if (!g)
+    return -1;
+  switch (what) {
+  case 0: return g->E;
+  case 1: return g->Np;
+  case 2: return g->V;
+  case 3: return static_cast<int64_t>(g->state_size());
+  case 4: return g->F;
+  case 5: return static_cast<int64_t>(g->device_bytes);
+  case 6: return g->lanes;
+  case 7:
+    const_cast<wasl_hip_engine *>(g)->build_stencil_host();
+    return g->stencil.checks();
+  default: return -1;
+  }
+}
+
+int wasl_hip_engine_trace(const wasl_hip_engine *g, int32_t *donors, double *basis) {
+  return guarded([&] {
+    if (!g || g->dim != 2)
+      throw std::runtime_error("engine: periodic traces exist for triangle meshes");
+    const size_t P = g->p + 1;
+    if (g->trace.donors.empty()) {
+      std::fill(donors, donors + g->F * P, -1);
+      std::fill(basis, basis + g->F * P * g->Np, 0.0);
+      return;
+    }
+    std::copy(g->trace.donors.begin(), g->trace.donors.end(), donors);
+    std::copy(g->trace.basis.begin(), g->trace.basis.end(), basis);
+  });
+}
+
+int wasl_hip_engine_stencil(const wasl_hip_engine *cg, int32_t *neighbors, double *transforms,
+                            double *normals, double *check_basis) {
+  return guarded([&] {
+    if (!cg || cg->dim != 2)
+      throw std::runtime_error("engine: limiter stencils exist for triangle meshes");
+    auto *g = const_cast<wasl_hip_engine *>(cg);
+    g->build_stencil_host();
+    const whip::LimiterStencil &s = g->stencil;
+    if (neighbors)
+      std::copy(s.neighbors.begin(), s.neighbors.end(), neighbors);
+    if (transforms)
+      std::copy(s.transforms.begin(), s.transforms.end(), transforms);
+    if (normals)
+      for (size_t e = 0; e < s.nx.size(); ++e) {
+        normals[2 * e] = s.nx[e];
+        normals[2 * e + 1] = s.ny[e];
+      }
+    if (check_basis)
+      std::copy(s.check.begin(), s.check.end(), check_basis);
+  });
+}
+
+int wasl_hip_project(wasl_hip_engine *g, int ic, const double *params, int64_t param_count,
+                     double *state_out) {
+  return guarded([&] {
+    if (!g)
+      throw std::runtime_error("engine: null argument");
+    const std::vector<double> U =
+        whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, ic, params, param_count);
+    HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
+                            g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    if (state_out)
+      std::copy(U.begin(), U.end(), state_out);
+  });
+}
+
+int wasl_hip_upload_state(wasl_hip_engine *g, const double *state) {
+  return guarded([&] {
+    HIPCHECK(hipMemcpyAsync(g->u.p, state, g->state_size() * sizeof(double),
+                            hipMemcpyHostToDevice, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  });
+}
+
+int wasl_hip_download_state(wasl_hip_engine *g, double *state) {
+  return guarded([&] {
+    HIPCHECK(hipMemcpyAsync(state, g->u.p, g->state_size() * sizeof(double),
+                            hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  });
+}
+
+int wasl_hip_residual(wasl_hip_engine *g, const double *state, double *rhs) {
+  return guarded([&] {
+    const size_t n = g->state_size() * sizeof(double);
+    HIPCHECK(hipMemcpyAsync(g->w1.p, state, n, hipMemcpyHostToDevice, g->stream));
+    whip::StageArgs a{};
+    a.uin = g->w1.as<double>();
+    a.un = g->w1.as<double>();
+    a.out = g->w2.as<double>();
+    a.cb = 1.0;
+    a.residual_only = 1;
+    a.list = g->list.as<int>();
+    a.counters = g->counters.as<int>();
+    launch_stage(*g, a);
+    HIPCHECK(hipMemcpyAsync(rhs, g->w2.p, n, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  });
+}
+
+int wasl_hip_limit(wasl_hip_engine *g, uint8_t *flags) {
+  return guarded([&] {
+    if (g->dim != 2)
+      throw std::runtime_error("engine: the limiter applies to 2D Euler");
+    if (g->p == 0) {
+      if (flags)
+        std::fill(flags, flags + g->E, 0);
+      return;
+    }
+    if (!g->lnbr.p)
+      throw std::runtime_error("engine: create the engine with limiting on to limit");
+    HIPCHECK(hipMemsetAsync(g->counters.as<int>() + 3, 0, sizeof(int), g->stream));
+    switch (g->p) {
+    case 1: launch_sensor_p<1>(*g, g->flags.as<uint8_t>()); break;
+    case 2: launch_sensor_p<2>(*g, g->flags.as<uint8_t>()); break;
+    default: launch_sensor_p<3>(*g, g->flags.as<uint8_t>()); break;
+    }
+    launch_limit(*g, g->u.as<double>(), 3, nullptr, 0);
+    if (flags)
+      HIPCHECK(hipMemcpyAsync(flags, g->flags.p, g->E, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    raise_device_error(*g);
+  });
+}
+
+int wasl_hip_stable_dt(wasl_hip_engine *g, double cfl, double *dt) {
+  return guarded([&] {
+    const double speed = resident_speed(*g);
+    if (g->dim == 1) {
+      *dt = speed == 0.0 ? 0.0 : cfl * g->hmin / (speed * (2.0 * g->p + 1.0));
+      return;
+    }
+    *dt = speed > 0.0 ? cfl * g->hmin / (speed * (2 * g->p + 1)) : 0.0;
+  });
+}
+
+int wasl_hip_step(wasl_hip_engine *g, double dt) {
+  return guarded([&] {
+    ensure_events(*g, 3);
+    int idx = 0;
+    HIPCHECK(hipEventRecord(g->ev[0], g->stream));
+    launch_step(*g, dt, nullptr, 0, false, idx);
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    raise_device_error(*g);
+  });
+}
+
+int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t max_steps,
+                     uint64_t history_interval, double *time, uint64_t *step, double *history,
+                     int64_t history_capacity, int64_t *history_rows) {
+  return guarded([&] {
+    if (!g || !time || !step)
+      throw std::runtime_error("engine: null argument");
+    if (history_interval == 0)
+      history_interval = ~0ull;
+    whip::RunCtl *h = g->ctl_host;
+    std::memset(h, 0, sizeof(*h));
+    h->end_time = end_time;
+    h->cfl_length = cfl * g->hmin;
+    h->order_factor = g->dim == 2 ? static_cast<double>(2 * g->p + 1) : (2.0 * g->p + 1.0);
+    h->max_steps = max_steps;
+    h->fixed_speed = g->dim == 1 ? 1 : 0;
+    h->s[0].time = *time;
+    h->s[0].step = *step;
+    if (g->dim == 1) {
+      const double a = std::fabs(g->num.wavespeed);
+      std::memcpy(&h->s[0].lambda, &a, sizeof(double));
+    }
+    HIPCHECK(hipMemcpyAsync(g->ctl.p, h, sizeof(*h), hipMemcpyHostToDevice, g->stream));
+    whip::RunCtl *dctl = g->ctl.as<whip::RunCtl>();
+    if (g->dim == 2)
+      hipLaunchKernelGGL(whip::cfl_speed, dim3((g->E + 255) / 256), dim3(256), 0, g->stream,
+                         g->u.as<double>(), g->E, g->Np, g->num.heat_capacity_ratio, dctl, 0);
+    int64_t rows = 0;
+    const auto record = [&](uint64_t st, double t, double dt) {
+      if (history && rows < history_capacity) {
+        const double m = mass_of(*g);
+        history[4 * rows] = static_cast<double>(st);
+        history[4 * rows + 1] = t;
+        history[4 * rows + 2] = dt;
+        history[4 * rows + 3] = m;
+      }
+      ++rows;
+    };
+    if (history)
+      record(*step, *time, 0.0);
+
+    const bool timed = g->timing;
+    constexpr int kBatch = 64;
+    if (timed)
+      ensure_events(*g, 3 * kBatch);
+    int parity = 0;
+    uint64_t cur_step = *step;
+    double cur_time = *time;
+    double last_dt = 0.0;
+    double stage_ms = 0.0;
+    int64_t launches = 0;
+    hipEvent_t t0, t1;
+    HIPCHECK(hipEventCreate(&t0));
+    HIPCHECK(hipEventCreate(&t1));
+    HIPCHECK(hipEventRecord(t0, g->stream));
+    while (true) {
+      uint64_t batch = kBatch;
+      if (history) {
+        const uint64_t to_next = history_interval - (cur_step % history_interval);
+        batch = std::min<uint64_t>(batch, to_next);
+      }
+      int idx = 0;
+      for (uint64_t i = 0; i < batch; ++i) {
+        launch_step(*g, 0.0, dctl, parity, timed, idx);
+        parity ^= 1;
+      }
+      HIPCHECK(hipMemcpyAsync(h, g->ctl.p, sizeof(*h), hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      if (timed) {
+        stage_ms += stage_time_ms(*g, idx);
+        launches += idx;
+      }
+      if (h->err)
+        break;
+      const whip::StepSlot &s = h->s[parity];
+      const bool advanced = s.step != cur_step;
+      if (advanced)
+        last_dt = h->last_dt;
+      cur_step = s.step;
+      cur_time = s.time;
+      const bool done = !(cur_time < end_time && cur_step < max_steps);
+      if (history && advanced && (cur_step % history_interval == 0 || done))
+        record(cur_step, cur_time, last_dt);
+      if (done)
+        break;
+    }
+    HIPCHECK(hipEventRecord(t1, g->stream));
+    HIPCHECK(hipEventSynchronize(t1));
+    float wall = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&wall, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    g->last_wall_ms = wall;
+    g->last_stage_ms = stage_ms;
+    g->last_stage_launches = launches;
+    if (h->err) {
+      const int code = h->err;
+      HIPCHECK(hipMemsetAsync(g->counters.p, 0, g->counters.bytes, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      throw SimulationError(device_error_text(code));
+    }
+    raise_device_error(*g);
+    *time = cur_time;
+    *step = cur_step;
+    if (history_rows)
+      *history_rows = rows;
+    // time and step are reported even when the run stops at max_steps
+    if (cur_time < end_time)
+      throw SimulationError("simulation: max_steps reached before end_time");
+  });
+}
+
+int wasl_hip_mass(wasl_hip_engine *g, double *mass) {
+  return guarded([&] { *mass = mass_of(*g); });
+}
+
+int wasl_hip_cell_averages(wasl_hip_engine *g, double *averages) {
+  return guarded([&] {
+    std::vector<double> U(g->state_size());
+    HIPCHECK(hipMemcpyAsync(U.data(), g->u.p, U.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    const double s = 1.0 / std::sqrt(2.0);
+    const size_t E = g->E;
+    if (g->dim == 2) {
+      for (size_t e = 0; e < E; ++e)
+        for (int v = 0; v < 4; ++v)
+          averages[e * 4 + v] = s * U[(static_cast<size_t>(v) * g->Np) * E + e];
+    } else {
+      for (size_t e = 0; e < E; ++e)
+        averages[e] = U[e * g->Np] / std::sqrt(2.0);
+    }
+  });
+}
+
+int wasl_hip_last_timing(const wasl_hip_engine *g, double *wall_ms, double *stage_ms,
+                         int64_t *stage_launches) {
+  return guarded([&] {
+    if (wall_ms)
+      *wall_ms = g->last_wall_ms;
+    if (stage_ms)
+      *stage_ms = g->last_stage_ms;
+    if (stage_launches)
+      *stage_launches = g->last_stage_launches;
+  });
+}
+
+// Not part of the reference interface: switches per-stage HIP event timing
+// for wasl_hip_advance (used by the benchmark to time the stage kernel).
+int wasl_hip_set_timing(wasl_hip_engine *g, int on) {
+  return guarded([&] { g->timing = on != 0; });
+}
+
+}  // extern "C"

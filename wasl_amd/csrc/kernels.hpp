@@ -1,0 +1,994 @@
+// Device code of the wasl_hip engine (gfx950, wave64).
+//
+// Fused SSP-RK3 stage for 2D Euler on triangles: one kernel evaluates the
+// modal->quadrature volume integral, the HLLC face fluxes with lifting (faces
+// gathered per element through precomputed edge records, no atomics), the
+// inverse mass 1/detJ and the Shu-Osher combination, and optionally the
+// troubled-cell sensor and the CFL speed of the new state.
+//
+// Work split: LANES consecutive lanes share one element. Volume points are
+// dealt round-robin to the lanes, edges likewise; each lane accumulates a
+// partial right-hand side for all variables and modes in registers and the
+// partials are summed with quad DPP moves. All reference-element tables sit
+// in LDS. State layout is the reference's U[variable][mode][element], so
+// every coefficient load is coalesced over the elements of a wave.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace whip {
+
+// ---------------------------------------------------------------- sizes ----
+template <int P> struct Tri {
+  static constexpr int Np = (P + 1) * (P + 2) / 2;
+  static constexpr int NQ = (P + 3) * (P + 3);          // volume rule of order p+2
+  static constexpr int NE = P + 1;                      // points per edge
+  static constexpr int NC = (P + 1) * (P + 1) + 3 * (P + 1);  // positivity check points
+  // Table offsets (doubles) inside one contiguous block.
+  static constexpr int kBvol = 0;
+  static constexpr int kWx = kBvol + NQ * Np;
+  static constexpr int kWy = kWx + NQ * Np;
+  static constexpr int kBl = kWy + NQ * Np;              // [3][NE][Np]
+  static constexpr int kBr = kBl + 3 * NE * Np;          // [3][3][2][NE][Np]
+  static constexpr int kWe = kBr + 18 * NE * Np;         // [3][NE]
+  static constexpr int kTable = kWe + 3 * NE;
+};
+
+template <int P> struct Line {
+  static constexpr int Np = P + 1;
+  static constexpr int NQ = P > 0 ? P : 1;
+  static constexpr int kB = 0;                 // [NQ][Np]
+  static constexpr int kW = kB + NQ * Np;      // [NQ][Np] weight * derivative
+  static constexpr int kL = kW + NQ * Np;      // [Np] phi(-1)
+  static constexpr int kR = kL + Np;           // [Np] phi(+1)
+  static constexpr int kTable = kR + Np;
+};
+
+// Edge-record meta bits.
+constexpr int kMetaLeft = 1;        // this element is the face's left side
+constexpr int kMetaBoundary = 64;   // boundary face (no right element)
+__host__ __device__ inline int meta_kl(int m) { return (m >> 1) & 3; }
+__host__ __device__ inline int meta_kr(int m) { return (m >> 3) & 3; }
+__host__ __device__ inline int meta_flip(int m) { return (m >> 5) & 1; }
+__host__ __device__ inline int meta_slot(int m) { return m >> 8; }
+
+// Error codes raised on the device; the host maps them to the reference texts.
+enum : int {
+  kErrNone = 0,
+  kErrUnphysical = 1,   // simulation: Euler cell average is not physical
+  kErrPositivity = 2,   // Euler positivity limiter: cell average is below a floor
+  kErrTimestep = 3,     // simulation: timestep cannot advance time
+};
+
+// Per-step control, double-buffered by step parity so one kernel can read
+// the current step while writing the next.
+struct StepSlot {
+  double time;
+  double dt;
+  unsigned long long step;
+  unsigned long long lambda;  // max |u|+c over cell averages (bits of a positive double)
+  int active;
+  int bad;                    // an unphysical cell average was seen
+};
+
+struct RunCtl {
+  double end_time;
+  double cfl_length;          // cfl * h_min
+  double order_factor;        // 2p + 1
+  unsigned long long max_steps;
+  int err;
+  int fixed_speed;            // 1: lambda is a case constant (scalar advection)
+  double last_dt;             // dt of the last step taken
+  StepSlot s[2];
+};
+
+struct EulerMesh {
+  int E;
+  const double *geo;          // [5][E]: A B C D 1/detJ with F_xi = A Fx + B Fy, F_eta = C Fx + D Fy
+  const int *nbr;             // [3][E] neighbour element or -1
+  const int *meta;            // [3][E]
+  const double *fn;           // [3][3][E]: nx, ny, edge scale (face values, from the left side)
+  const int *bdonor;          // [NB][NE] periodic donors (or null: transmissive)
+  const double *bbasis;       // [NB][NE][Np]
+  const double *table;        // Tri<P>::kTable doubles
+  double gamma;
+};
+
+struct StageArgs {
+  const double *uin;          // stage input, residual evaluated here
+  const double *un;           // step start (RK combination), may alias out
+  double *out;                // stage output, or R when residual_only
+  double ca, cb;              // out = ca*un + cb*(uin + dt*R)
+  double dt;                  // used when ctl == nullptr
+  RunCtl *ctl;                // device-driven time stepping, or nullptr
+  int parity;
+  int stage;                  // 0, 1, 2
+  int residual_only;
+  int want_lambda;            // accumulate the CFL speed of the output
+  // limiter hooks
+  int sensor;                 // compute the troubled-cell sensor of the output
+  double threshold;
+  int highest_begin;
+  uint8_t *flags;             // [E] or null
+  int *list;                  // troubled elements
+  int *counters;              // [3] per-stage troubled counts
+};
+
+// ------------------------------------------------------------- helpers ----
+
+template <int CTRL> __device__ __forceinline__ double dpp(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// quad_perm controls: lane i of each quad reads lane perm[i].
+constexpr int kQuadXor1 = 0xB1;   // [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;   // [2,3,0,1]
+template <int SRC> constexpr int quad_bcast() { return SRC | (SRC << 2) | (SRC << 4) | (SRC << 6); }
+
+template <int LANES> __device__ __forceinline__ double lane_sum(double x) {
+  if constexpr (LANES >= 2)
+    x += dpp<kQuadXor1>(x);
+  if constexpr (LANES >= 4)
+    x += dpp<kQuadXor2>(x);
+  return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+  for (int o = 32; o > 0; o >>= 1)
+    x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+__device__ __forceinline__ void raise_error(RunCtl *ctl, int *err, int code) {
+  int *slot = ctl ? &ctl->err : err;
+  if (slot)
+    atomicCAS(slot, 0, code);
+}
+
+// Ideal-gas pointwise physics (equations/euler/primitive.hpp, flux.hpp).
+struct C4 {
+  double r, mx, my, E;
+};
+
+__device__ __forceinline__ void normal_flux(double g1, const C4 &s, double nx, double ny,
+                                            double &u, double &v, double &p, C4 &f) {
+  const double inv = 1.0 / s.r;
+  u = s.mx * inv;
+  v = s.my * inv;
+  p = g1 * (s.E - 0.5 * (s.mx * u + s.my * v));
+  const double un = u * nx + v * ny;
+  f.r = s.r * un;
+  f.mx = s.mx * un + p * nx;
+  f.my = s.my * un + p * ny;
+  f.E = (s.E + p) * un;
+}
+
+// HLLC flux with Davis wave-speed bounds, evaluated (left, right, n_left)
+// (flux/hllc.hpp:40-74).
+__device__ inline C4 hllc(double gamma, const C4 &L, const C4 &R, double nx, double ny) {
+  const double g1 = gamma - 1.0;
+  double ul, vl, pl, ur, vr, pr;
+  C4 fl, fr;
+  normal_flux(g1, L, nx, ny, ul, vl, pl, fl);
+  normal_flux(g1, R, nx, ny, ur, vr, pr, fr);
+  const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
+  const double cl = sqrt(gamma * pl / L.r), cr = sqrt(gamma * pr / R.r);
+  const double sl = fmin(unl - cl, unr - cr);
+  const double sr = fmax(unl + cl, unr + cr);
+  if (sl >= 0.0)
+    return fl;
+  if (sr <= 0.0)
+    return fr;
+  const double sm = (pr - pl + L.r * unl * (sl - unl) - R.r * unr * (sr - unr)) /
+                    (L.r * (sl - unl) - R.r * (sr - unr));
+  const bool left = sm >= 0.0;
+  const C4 &S = left ? L : R;
+  const C4 &F = left ? fl : fr;
+  const double s = left ? sl : sr;
+  const double un = left ? unl : unr;
+  const double uu = left ? ul : ur, vv = left ? vl : vr, pp = left ? pl : pr;
+  const double rs = S.r * (s - un) / (s - sm);
+  const double shift = sm - un;
+  const double es = S.E / S.r + shift * (sm + pp / (S.r * (s - un)));
+  C4 o;
+  o.r = F.r + s * (rs - S.r);
+  o.mx = F.mx + s * (rs * (uu + shift * nx) - S.mx);
+  o.my = F.my + s * (rs * (vv + shift * ny) - S.my);
+  o.E = F.E + s * (rs * es - S.E);
+  return o;
+}
+
+// Device-driven time stepping (serial_case_runner.cpp:107-142): stage 0 of a
+// step decides whether the step runs (time < end_time, step < max_steps),
+// computes dt = min(cfl h / (lambda (2p+1)), end_time - time) from the speed
+// gathered at the end of the previous step, and publishes it for stages 1-2.
+// Every block evaluates the same decision from the same data, so the block
+// exits are uniform. Returns false when the stage must not run.
+__device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double &dt) {
+  if (!ctl)
+    return true;
+  if (ctl->err)
+    return false;
+  StepSlot &cur = ctl->s[parity];
+  if (stage != 0) {
+    if (!cur.active)
+      return false;
+    dt = cur.dt;
+    return true;
+  }
+  const double time = cur.time;
+  const bool active = time < ctl->end_time && cur.step < ctl->max_steps;
+  if (!active) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      cur.active = 0;
+      StepSlot &nxt = ctl->s[parity ^ 1];
+      nxt.time = time;
+      nxt.step = cur.step;
+      nxt.lambda = cur.lambda;
+      nxt.bad = cur.bad;
+      nxt.active = 0;
+    }
+    return false;
+  }
+  if (cur.bad) {
+    if (threadIdx.x == 0)
+      raise_error(ctl, nullptr, kErrUnphysical);
+    return false;
+  }
+  const double speed = __longlong_as_double(static_cast<long long>(cur.lambda));
+  const double remaining = ctl->end_time - time;
+  const double stable = speed > 0.0 ? ctl->cfl_length / (speed * ctl->order_factor) : 0.0;
+  dt = fmin(stable > 0.0 ? stable : remaining, remaining);
+  if (dt <= 0.0 || time + dt == time) {
+    if (threadIdx.x == 0)
+      raise_error(ctl, nullptr, kErrTimestep);
+    return false;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cur.dt = dt;
+    cur.active = 1;
+    ctl->last_dt = dt;
+    StepSlot &nxt = ctl->s[parity ^ 1];
+    nxt.lambda = ctl->fixed_speed ? cur.lambda : 0ull;
+    nxt.bad = 0;
+  }
+  return true;
+}
+
+// The last stage of a step advances time and step count for the next step.
+__device__ inline void stage_epilogue(RunCtl *ctl, int parity, int stage, double dt) {
+  if (ctl && stage == 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+    StepSlot &cur = ctl->s[parity];
+    StepSlot &nxt = ctl->s[parity ^ 1];
+    nxt.time = cur.time + dt;
+    nxt.step = cur.step + 1;
+  }
+}
+
+// ------------------------------------------------------ 2D Euler stage ----
+
+template <int P, int LANES>
+__global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) {
+  using D = Tri<P>;
+  constexpr int Np = D::Np, NQ = D::NQ, NE = D::NE;
+  constexpr int PER_BLOCK = 256 / LANES;
+  __shared__ double tab[D::kTable];
+  __shared__ double red[4];
+
+  RunCtl *ctl = a.ctl;
+  double dt = a.dt;
+  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+    return;
+  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
+
+  for (int i = threadIdx.x; i < D::kTable; i += 256)
+    tab[i] = mesh.table[i];
+  __syncthreads();
+
+  const int E = mesh.E;
+  const int sub = threadIdx.x % LANES;
+  const int e_raw = blockIdx.x * PER_BLOCK + threadIdx.x / LANES;
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+  const double g = mesh.gamma, g1 = g - 1.0;
+
+  double c[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      c[v][m] = a.uin[(v * Np + m) * E + e];
+
+  double acc[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      acc[v][m] = 0.0;
+
+  // Volume: sum_q (W_xi[q][m] F_xi + W_eta[q][m] F_eta) with the reference
+  // fluxes F_xi = A Fx + B Fy, F_eta = C Fx + D Fy (= detJ J^-1 F).
+  if constexpr (P > 0) {
+    const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
+                 gD = mesh.geo[3 * E + e];
+    for (int it = 0; it < (NQ + LANES - 1) / LANES; ++it) {
+      const int q = it * LANES + sub;
+      if (q < NQ) {
+        const double *phi = tab + D::kBvol + q * Np;
+        double s[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double t = 0.0;
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+            t += c[v][m] * phi[m];
+          s[v] = t;
+        }
+        const double inv = 1.0 / s[0];
+        const double u = s[1] * inv, w = s[2] * inv;
+        const double p = g1 * (s[3] - 0.5 * (s[1] * u + s[2] * w));
+        const double fx[4] = {s[1], s[1] * u + p, s[2] * u, (s[3] + p) * u};
+        const double fy[4] = {s[2], s[1] * w, s[2] * w + p, (s[3] + p) * w};
+        const double *wx = tab + D::kWx + q * Np;
+        const double *wy = tab + D::kWy + q * Np;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double fxi = gA * fx[v] + gB * fy[v];
+          const double feta = gC * fx[v] + gD * fy[v];
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+            acc[v][m] += wx[m] * fxi + wy[m] * feta;
+        }
+      }
+    }
+  }
+
+  // Faces: each lane takes whole edges; traces on both sides use the left
+  // side's quadrature point so the two elements of a face see identical
+  // flux bits (exact conservation).
+  for (int k = sub; k < 3; k += LANES) {
+    const int meta = mesh.meta[k * E + e];
+    const int nb = mesh.nbr[k * E + e];
+    const double nx = mesh.fn[(3 * k) * E + e];
+    const double ny = mesh.fn[(3 * k + 1) * E + e];
+    const double scale = mesh.fn[(3 * k + 2) * E + e];
+    const bool is_left = meta & kMetaLeft;
+    const bool boundary = meta & kMetaBoundary;
+    const int kl = meta_kl(meta), kr = meta_kr(meta), fl = meta_flip(meta);
+    const double *bl = tab + D::kBl + kl * NE * Np;
+    const double *br = tab + D::kBr + ((kl * 3 + kr) * 2 + fl) * NE * Np;
+    const double *own = is_left ? bl : br;
+    const double *oth = is_left ? br : bl;
+    for (int q = 0; q < NE; ++q) {
+      C4 si, so;
+      double t[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          x += c[v][m] * own[q * Np + m];
+        t[v] = x;
+      }
+      si = {t[0], t[1], t[2], t[3]};
+      if (!boundary) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = 0.0;
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+            x += a.uin[(v * Np + m) * E + nb] * oth[q * Np + m];
+          t[v] = x;
+        }
+        so = {t[0], t[1], t[2], t[3]};
+      } else if (mesh.bdonor) {
+        const int slot = meta_slot(meta);
+        const int d = mesh.bdonor[slot * NE + q];
+        const double *db = mesh.bbasis + (slot * NE + q) * Np;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = 0.0;
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+            x += a.uin[(v * Np + m) * E + d] * db[m];
+          t[v] = x;
+        }
+        so = {t[0], t[1], t[2], t[3]};
+      } else {
+        so = si;  // transmissive: exterior = own trace
+      }
+      const C4 f = is_left ? hllc(g, si, so, nx, ny) : hllc(g, so, si, nx, ny);
+      const double wq = tab[D::kWe + kl * NE + q] * scale;
+      const double sgn = is_left ? -1.0 : 1.0;
+      const double val[4] = {sgn * (f.r * wq), sgn * (f.mx * wq), sgn * (f.my * wq),
+                             sgn * (f.E * wq)};
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          acc[v][m] += val[v] * own[q * Np + m];
+    }
+  }
+
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      acc[v][m] = lane_sum<LANES>(acc[v][m]);
+
+  // Finalise: lane `sub` owns variables [sub*4/LANES, (sub+1)*4/LANES).
+  const double invdet = mesh.geo[4 * E + e];
+  double out0[4] = {0.0, 0.0, 0.0, 0.0};
+  double dens[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m)
+    dens[m] = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    if (v * LANES / 4 != sub)
+      continue;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      const double r = acc[v][m] * invdet;
+      const int idx = (v * Np + m) * E + e;
+      double y;
+      if (a.residual_only) {
+        y = r;
+      } else {
+        y = a.cb * (c[v][m] + dt * r);
+        if (a.ca != 0.0)
+          y += a.ca * a.un[idx];
+      }
+      if (valid)
+        a.out[idx] = y;
+      if (m == 0)
+        out0[v] = y;
+      if (v == 0)
+        dens[m] = y;
+    }
+  }
+
+  // Gather the new cell averages to every lane of the element: only the
+  // owning lane holds a nonzero value, so the lane sum is exact.
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    out0[v] = lane_sum<LANES>(out0[v]);
+
+  if (a.sensor && !a.residual_only && sub == 0 && valid) {
+    // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
+    double total = 0.0, high = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      total += dens[m] * dens[m];
+      if (m >= a.highest_begin)
+        high += dens[m] * dens[m];
+    }
+    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
+    if (a.flags)
+      a.flags[e] = troubled ? 1 : 0;
+    if (troubled)
+      a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+  }
+
+  if (a.want_lambda && !a.residual_only) {
+    // |u| + c of the new cell average (cfl_timestep.cpp:33-47).
+    double speed = 0.0;
+    if (sub == 0 && valid) {
+      const double is2 = 1.0 / sqrt(2.0);
+      const double r = is2 * out0[0], mx = is2 * out0[1], my = is2 * out0[2], en = is2 * out0[3];
+      const double inv = 1.0 / r;
+      const double u = mx * inv, w = my * inv;
+      const double p = g1 * (en - 0.5 * (mx * u + my * w));
+      if (!(r > 0.0 && p > 0.0)) {
+        if (ctl)
+          ctl->s[a.parity ^ 1].bad = 1;
+        speed = 0.0;
+      } else {
+        speed = hypot(u, w) + sqrt(g * p / r);
+      }
+    }
+    speed = wave_max(speed);
+    if ((threadIdx.x & 63) == 0)
+      red[threadIdx.x >> 6] = speed;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      if (ctl)
+        atomicMax(&ctl->s[a.parity ^ 1].lambda,
+                  static_cast<unsigned long long>(__double_as_longlong(mx)));
+    }
+  }
+  stage_epilogue(ctl, a.parity, a.stage, dt);
+}
+
+// ------------------------------------------------- 1D scalar advection ----
+
+struct LineMeshDev {
+  int E;
+  const double *inv_jac;   // [E] 1 / (h/2)
+  const double *table;     // Line<P>::kTable
+  double wavespeed;
+  int left_kind, right_kind;  // periodic / inflow / outflow
+  double left_value, right_value;
+};
+
+__device__ __forceinline__ double upwind(double a, double l, double r) {
+  return a >= 0.0 ? a * l : a * r;
+}
+
+// One lane per element, state U[element][mode]. Face fluxes are those of the
+// reference's face loop (dg/scalar_advection/line_residual.hpp:76-103) with
+// exteriors from make_line_boundary_exterior (line_boundary_conditions.hpp:36-57).
+template <int P> __global__ __launch_bounds__(256) void line_stage(LineMeshDev mesh, StageArgs a) {
+  using D = Line<P>;
+  constexpr int Np = D::Np, NQ = D::NQ;
+  __shared__ double tab[D::kTable];
+  double dt = a.dt;
+  if (!stage_prologue(a.ctl, a.parity, a.stage, dt))
+    return;
+  for (int i = threadIdx.x; i < D::kTable; i += blockDim.x)
+    tab[i] = mesh.table[i];
+  __syncthreads();
+  const int E = mesh.E;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) {
+    const double sa = mesh.wavespeed;
+    double c[Np], acc[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      c[m] = a.uin[e * Np + m];
+      acc[m] = 0.0;
+    }
+    const double *bl = tab + D::kL, *br = tab + D::kR;
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double u = 0.0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          u += c[m] * tab[D::kB + q * Np + m];
+        const double f = sa * u;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          acc[m] += f * tab[D::kW + q * Np + m];
+      }
+    }
+    const auto trace = [&](int el, const double *b) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        t += a.uin[el * Np + m] * b[m];
+      return t;
+    };
+    double own_l = 0.0, own_r = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      own_l += c[m] * bl[m];
+      own_r += c[m] * br[m];
+    }
+    // left face of e
+    double fl;
+    if (e == 0) {
+      const double ext = mesh.left_kind == 1   ? mesh.left_value
+                         : mesh.left_kind == 2 ? own_l
+                                               : trace(E - 1, br);
+      fl = upwind(sa, ext, own_l);
+    } else {
+      fl = upwind(sa, trace(e - 1, br), own_l);
+    }
+    // right face of e
+    double fr;
+    if (e == E - 1) {
+      const double ext = mesh.right_kind == 1   ? mesh.right_value
+                         : mesh.right_kind == 2 ? own_r
+                                                : trace(0, bl);
+      fr = upwind(sa, own_r, ext);
+    } else {
+      fr = upwind(sa, own_r, trace(e + 1, bl));
+    }
+    const double ij = mesh.inv_jac[e];
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      const double r = (acc[m] + bl[m] * fl - br[m] * fr) * ij;
+      double y;
+      if (a.residual_only) {
+        y = r;
+      } else {
+        y = a.cb * (c[m] + dt * r);
+        if (a.ca != 0.0)
+          y += a.ca * a.un[e * Np + m];
+      }
+      a.out[e * Np + m] = y;
+    }
+  }
+  stage_epilogue(a.ctl, a.parity, a.stage, dt);
+}
+
+// ------------------------------------------------------- reductions ----
+
+// CFL speed of the cell averages of U[4][Np][E] into ctl->s[parity].lambda
+// (cfl_timestep.cpp:33-47); an unphysical average sets the slot's bad flag.
+__global__ __launch_bounds__(256) void cfl_speed(const double *U, int E, int Np, double gamma,
+                                                 RunCtl *ctl, int parity) {
+  __shared__ double red[4];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double speed = 0.0;
+  if (e < E) {
+    const double is2 = 1.0 / sqrt(2.0);
+    const double r = is2 * U[e], mx = is2 * U[Np * E + e], my = is2 * U[2 * Np * E + e],
+                 en = is2 * U[3 * Np * E + e];
+    const double inv = 1.0 / r;
+    const double u = mx * inv, w = my * inv;
+    const double p = (gamma - 1.0) * (en - 0.5 * (mx * u + my * w));
+    if (!(r > 0.0 && p > 0.0))
+      ctl->s[parity].bad = 1;
+    else
+      speed = hypot(u, w) + sqrt(gamma * p / r);
+  }
+  speed = wave_max(speed);
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = speed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&ctl->s[parity].lambda, static_cast<unsigned long long>(__double_as_longlong(mx)));
+  }
+}
+
+// Neumaier-compensated sum of weight[e] * U[e * stride] in fixed order: each
+// block sums a contiguous chunk, a second pass sums the block partials.
+struct KSum {
+  double s, c;
+};
+__device__ __forceinline__ void ksum_add(KSum &k, double x) {
+  const double t = k.s + x;
+  k.c += fabs(k.s) >= fabs(x) ? (k.s - t) + x : (x - t) + k.s;
+  k.s = t;
+}
+
+__global__ __launch_bounds__(256) void mass_partials(const double *U, int stride,
+                                                     const double *weight, int E,
+                                                     double *partials) {
+  __shared__ double ss[256], sc[256];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  KSum k{0.0, 0.0};
+  if (e < E)
+    ksum_add(k, weight[e] * U[static_cast<size_t>(e) * stride]);
+  ss[threadIdx.x] = k.s;
+  sc[threadIdx.x] = k.c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum b{0.0, 0.0};
+    for (int i = 0; i < blockDim.x; ++i) {
+      ksum_add(b, ss[i]);
+      ksum_add(b, sc[i]);
+    }
+    partials[2 * blockIdx.x] = b.s;
+    partials[2 * blockIdx.x + 1] = b.c;
+  }
+}
+
+__global__ void mass_finish(const double *partials, int blocks, double *result) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    KSum b{0.0, 0.0};
+    for (int i = 0; i < blocks; ++i) {
+      ksum_add(b, partials[2 * i]);
+      ksum_add(b, partials[2 * i + 1]);
+    }
+    *result = b.s + b.c;
+  }
+}
+
+// --------------------------------------------------------------- limiter ----
+
+struct LimiterDev {
+  int E;
+  const int *nbr;        // [E][3]
+  const double *T;       // [E][3][Np][Np]
+  const double *nrm;     // [E][2]
+  const double *check;   // [NC][Np]
+  double gamma, density_floor, pressure_floor;
+};
+
+// Standalone troubled-cell sensor over density modes.
+template <int P>
+__global__ __launch_bounds__(256) void sensor(const double *U, int E, double threshold,
+                                              int highest_begin, uint8_t *flags, int *list,
+                                              int *count) {
+  constexpr int Np = Tri<P>::Np;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E)
+    return;
+  double total = 0.0, high = 0.0;
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    const double v = U[m * E + e];
+    total += v * v;
+    if (m >= highest_begin)
+      high += v * v;
+  }
+  const bool troubled = P > 0 && !(total == 0.0 || high / total <= threshold);
+  if (flags)
+    flags[e] = troubled ? 1 : 0;
+  if (troubled)
+    list[atomicAdd(count, 1)] = e;
+}
+
+// Characteristic frame frozen at a cell average (limiting/weno/euler_characteristic.hpp).
+struct CharFrame {
+  double nx, ny, un, ut, c, ke, factor, H;
+};
+
+__device__ inline CharFrame char_frame(double g, const double avg[4], double nx, double ny) {
+  CharFrame f;
+  const double inv = 1.0 / avg[0];
+  const double u = avg[1] * inv, v = avg[2] * inv;
+  const double p = (g - 1.0) * (avg[3] - 0.5 * (avg[1] * u + avg[2] * v));
+  f.nx = nx;
+  f.ny = ny;
+  f.un = u * nx + v * ny;
+  f.ut = -u * ny + v * nx;
+  const double c2 = g * p / avg[0];
+  f.c = sqrt(c2);
+  f.ke = 0.5 * (f.un * f.un + f.ut * f.ut);
+  f.factor = (g - 1.0) / c2;
+  f.H = (avg[3] + p) / avg[0];
+  return f;
+}
+
+__device__ __forceinline__ void to_char(const CharFrame &f, double r, double mx, double my,
+                                        double E, double w[4]) {
+  const double mn = mx * f.nx + my * f.ny;
+  const double mt = -mx * f.ny + my * f.nx;
+  const double common = f.factor * (f.ke * r - f.un * mn - f.ut * mt + E);
+  const double diff = (f.un * r - mn) / f.c;
+  w[0] = 0.5 * (common + diff);
+  w[1] = r - common;
+  w[2] = mt - f.ut * r;
+  w[3] = 0.5 * (common - diff);
+}
+
+__device__ __forceinline__ void from_char(const CharFrame &f, const double w[4], double out[4]) {
+  const double r = w[0] + w[1] + w[3];
+  const double mn = (f.un - f.c) * w[0] + f.un * w[1] + (f.un + f.c) * w[3];
+  const double mt = f.ut * r + w[2];
+  out[0] = r;
+  out[1] = f.nx * mn - f.ny * mt;
+  out[2] = f.ny * mn + f.nx * mt;
+  out[3] = (f.H - f.un * f.c) * w[0] + f.ke * w[1] + f.ut * w[2] + (f.H + f.un * f.c) * w[3];
+}
+
+__device__ __forceinline__ double pressure_of(double g, double r, double mx, double my, double E) {
+  const double inv = 1.0 / r;
+  const double u = mx * inv, v = my * inv;
+  return (g - 1.0) * (E - 0.5 * (mx * u + my * v));
+}
+
+// Troubled elements from the list: characteristic simple WENO against the
+// transformed neighbours (all read from the unlimited state), then the
+// Zhang-Shu positivity limiter in FP64 (euler_modal_limiter.cpp:81-97,
+// euler_characteristic_weno.hpp:17-67, simple_weno.hpp:21-48,
+// euler_positivity.hpp:63-102). Results go to side[idx] for scatter().
+template <int P>
+__global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double *U, double *side,
+                                                     const int *list, const int *count,
+                                                     RunCtl *ctl, int parity, int *err) {
+  constexpr int Np = Tri<P>::Np, NC = Tri<P>::NC;
+  if (ctl && (ctl->err || !ctl->s[parity].active))
+    return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= *count)
+    return;
+  const int e = list[idx];
+  const int E = L.E;
+  const double g = L.gamma;
+  double t[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      t[v][m] = U[(v * Np + m) * E + e];
+  const double s2 = sqrt(2.0);
+  const double avg[4] = {t[0][0] / s2, t[1][0] / s2, t[2][0] / s2, t[3][0] / s2};
+  const CharFrame f = char_frame(g, avg, L.nrm[2 * e], L.nrm[2 * e + 1]);
+
+  double tc[4][Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    double w[4];
+    to_char(f, t[0][m], t[1][m], t[2][m], t[3][m], w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      tc[k][m] = w[k];
+  }
+  int nbs[3];
+  int K = 0;
+  for (int k = 0; k < 3; ++k) {
+    const int nb = L.nbr[3 * e + k];
+    if (nb >= 0)
+      nbs[K++] = k;
+  }
+  double lim[4][Np];
+  if (K == 0) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        lim[w][m] = tc[w][m];
+  } else {
+    constexpr double eps = 1e-12;
+    double w0[4], wk[3][4], wsum[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      double b = 0.0;
+#pragma unroll
+      for (int m = 1; m < Np; ++m)
+        b += tc[w][m] * tc[w][m];
+      const double d = eps + b;
+      w0[w] = 0.5 / (d * d);
+      wsum[w] = w0[w];
+    }
+    const double ideal = 0.5 / static_cast<double>(K);
+    // Candidate j in characteristic variables: T_j U_nb with the element's own mean.
+    const auto candidate = [&](int j, double cc[4][Np]) {
+      const int k = nbs[j];
+      const int nb = L.nbr[3 * e + k];
+      const double *M = L.T + (static_cast<size_t>(e) * 3 + k) * Np * Np;
+      double cv[4][Np];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double src[Np];
+#pragma unroll
+        for (int sm = 0; sm < Np; ++sm)
+          src[sm] = U[(v * Np + sm) * E + nb];
+#pragma unroll
+        for (int tm = 0; tm < Np; ++tm) {
+          double x = 0.0;
+#pragma unroll
+          for (int sm = 0; sm < Np; ++sm)
+            x += M[tm * Np + sm] * src[sm];
+          cv[v][tm] = x;
+        }
+        cv[v][0] = t[v][0];
+      }
+#pragma unroll
+      for (int m = 0; m < Np; ++m) {
+        double w[4];
+        to_char(f, cv[0][m], cv[1][m], cv[2][m], cv[3][m], w);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2)
+          cc[k2][m] = w[k2];
+      }
+    };
+    double cc[4][Np];
+    for (int j = 0; j < K; ++j) {
+      candidate(j, cc);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        double b = 0.0;
+#pragma unroll
+        for (int m = 1; m < Np; ++m)
+          b += cc[w][m] * cc[w][m];
+        const double d = eps + b;
+        wk[j][w] = ideal / (d * d);
+        wsum[w] += wk[j][w];
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      lim[w][0] = tc[w][0];
+#pragma unroll
+      for (int m = 1; m < Np; ++m)
+        lim[w][m] = w0[w] * tc[w][m];
+    }
+    for (int j = 0; j < K; ++j) {
+      candidate(j, cc);
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (int m = 1; m < Np; ++m)
+          lim[w][m] += wk[j][w] * cc[w][m];
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+      for (int m = 1; m < Np; ++m)
+        lim[w][m] = lim[w][m] / wsum[w];
+  }
+  // Back to conserved variables, restoring every mean.
+  double o[4][Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    const double w[4] = {lim[0][m], lim[1][m], lim[2][m], lim[3][m]};
+    double out[4];
+    from_char(f, w, out);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      o[v][m] = out[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    o[v][0] = t[v][0];
+
+  // Positivity: density, then pressure by 64-step bisection, both scaling the
+  // high modes toward the mean.
+  const double is2 = 1.0 / sqrt(2.0);
+  const double a[4] = {o[0][0] * is2, o[1][0] * is2, o[2][0] * is2, o[3][0] * is2};
+  if (a[0] <= L.density_floor || pressure_of(g, a[0], a[1], a[2], a[3]) <= L.pressure_floor) {
+    raise_error(ctl, err, kErrPositivity);
+    return;
+  }
+  double ds = 1.0;
+  for (int q = 0; q < NC; ++q) {
+    const double *b = L.check + q * Np;
+    double r = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      r += o[0][m] * b[m];
+    if (r < L.density_floor)
+      ds = fmin(ds, (a[0] - L.density_floor) / (a[0] - r));
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 1; m < Np; ++m)
+      o[v][m] *= ds;
+  double ps = 1.0;
+  for (int q = 0; q < NC; ++q) {
+    const double *b = L.check + q * Np;
+    double s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double x = 0.0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        x += o[v][m] * b[m];
+      s[v] = x;
+    }
+    if (pressure_of(g, s[0], s[1], s[2], s[3]) >= L.pressure_floor)
+      continue;
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 64; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      const double r = a[0] + mid * (s[0] - a[0]);
+      const double mx = a[1] + mid * (s[1] - a[1]);
+      const double my = a[2] + mid * (s[2] - a[2]);
+      const double en = a[3] + mid * (s[3] - a[3]);
+      if (pressure_of(g, r, mx, my, en) >= L.pressure_floor)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    ps = fmin(ps, lo);
+  }
+  double *dst = side + static_cast<size_t>(idx) * 4 * Np;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      dst[v * Np + m] = m == 0 ? o[v][m] : o[v][m] * ps;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
+                                                       const int *list, const int *count,
+                                                       RunCtl *ctl, int parity) {
+  constexpr int Np = Tri<P>::Np;
+  if (ctl && (ctl->err || !ctl->s[parity].active))
+    return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= *count)
+    return;
+  const int e = list[idx];
+  const double *src = side + static_cast<size_t>(idx) * 4 * Np;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      U[(v * Np + m) * E + e] = src[v * Np + m];
+}
+
+}  // namespace whip
