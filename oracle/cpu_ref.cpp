@@ -1566,4 +1566,123 @@ int orc_run(orc_case *c, double cfl, int limiting, double threshold, double dens
   });
 }
 
+// ---- function-level hooks ----
+
+int orc_basis(int kind, int p, double xi, double eta, int gradient, double *values, double *gx,
+              double *ge) {
+  return guarded([&] {
+    if (kind == 0) {
+      legendre(xi, p, values);
+      if (gradient)
+        legendre_gradient(xi, p, gx);
+    } else {
+      dubiner(xi, eta, p, values);
+      if (gradient)
+        dubiner_gradient(xi, eta, p, gx, ge);
+    }
+  });
+}
+
+int orc_rule(int kind, int n, int edge, double *a, double *b, double *w) {
+  int count = -1;
+  if (guarded([&] {
+        if (kind == 0) {
+          const Rule r = gauss(n);
+          count = r.size();
+          std::copy(r.x.begin(), r.x.end(), a);
+          std::copy(r.w.begin(), r.w.end(), w);
+        } else if (kind == 1) {
+          const TriRule r = tri_rule(n);
+          count = r.size();
+          std::copy(r.xi.begin(), r.xi.end(), a);
+          std::copy(r.eta.begin(), r.eta.end(), b);
+          std::copy(r.w.begin(), r.w.end(), w);
+        } else {
+          const EdgeRule r = edge_rules(n).at(static_cast<size_t>(edge));
+          count = static_cast<int>(r.xi.size());
+          std::copy(r.xi.begin(), r.xi.end(), a);
+          std::copy(r.eta.begin(), r.eta.end(), b);
+          std::copy(r.w.begin(), r.w.end(), w);
+        }
+      }) != 0)
+    return -1;
+  return count;
+}
+
+int orc_euler_flux(double gamma, const double *s, double *fx, double *fy) {
+  return guarded([&] {
+    const Cons u{s[0], s[1], s[2], s[3]};
+    const Cons a = flux_x(gamma, u), b = flux_y(gamma, u);
+    for (int v = 0; v < 4; ++v) {
+      fx[v] = a[v];
+      fy[v] = b[v];
+    }
+  });
+}
+
+int orc_hllc(double gamma, const double *l, const double *r, double nx, double ny,
+             double *flux) {
+  return guarded([&] {
+    const Cons f = hllc(gamma, {l[0], l[1], l[2], l[3]}, {r[0], r[1], r[2], r[3]}, nx, ny);
+    for (int v = 0; v < 4; ++v)
+      flux[v] = f[v];
+  });
+}
+
+int orc_conserved(double gamma, const double *w, double *s) {
+  return guarded([&] {
+    const Cons u = conserved(gamma, {w[0], w[1], w[2], w[3]});
+    for (int v = 0; v < 4; ++v)
+      s[v] = u[v];
+  });
+}
+
+int orc_characteristic(double gamma, const double *avg, const double *d, double nx, double ny,
+                       double *waves, double *back) {
+  return guarded([&] {
+    const Cons a{avg[0], avg[1], avg[2], avg[3]};
+    to_characteristic(gamma, a, {d[0], d[1], d[2], d[3]}, nx, ny, waves);
+    const Cons b = from_characteristic(gamma, a, waves, nx, ny);
+    for (int v = 0; v < 4; ++v)
+      back[v] = b[v];
+  });
+}
+
+int orc_simple_weno(const double *target, const double *candidates, int K, int n,
+                    double *limited) {
+  return guarded([&] { simple_weno(target, candidates, K, n, limited); });
+}
+
+int orc_characteristic_weno(double gamma, const double *avg, double nx, double ny,
+                            const double *target, const double *candidates, int K, int n,
+                            double *limited) {
+  return guarded([&] {
+    characteristic_weno(gamma, {avg[0], avg[1], avg[2], avg[3]}, nx, ny, target, candidates, K,
+                        n, limited);
+  });
+}
+
+int orc_positivity(double gamma, double *c, int n, const double *basis, int points, double rf,
+                   double pf, double *scale) {
+  return guarded([&] { *scale = positivity(gamma, c, n, basis, points, rf, pf); });
+}
+
+// include/wasl/limiting/troubled_cell/modal_decay.hpp:36-47
+int orc_modal_decay_troubled(const double *c, int p, double threshold, int *troubled) {
+  return guarded([&] {
+    double ratio = 0.0;
+    if (p >= 1) {
+      const int n = tri_modes(p), hb = tri_modes(p - 1);
+      double total = 0.0, high = 0.0;
+      for (int m = 0; m < n; ++m) {
+        total += c[m] * c[m];
+        if (m >= hb)
+          high += c[m] * c[m];
+      }
+      ratio = total > 0.0 ? high / total : 0.0;
+    }
+    *troubled = ratio > threshold ? 1 : 0;
+  });
+}
+
 }  // extern "C"

@@ -72,9 +72,21 @@ def load(kind: str = "port") -> C.CDLL:
     lib.orc_project.argtypes = [_P, C.c_int, _pd, _i64, _pd]
     lib.orc_run.argtypes = [_P, _d, C.c_int, _d, _d, _d, _d, C.c_uint64, _d, C.c_uint64,
                             C.c_uint64, _pd, _pu64, C.POINTER(_d), _pd, _i64, C.POINTER(_i64)]
+    lib.orc_basis.argtypes = [C.c_int, C.c_int, _d, _d, C.c_int, _pd, _pd, _pd]
+    lib.orc_rule.argtypes = [C.c_int, C.c_int, C.c_int, _pd, _pd, _pd]
+    lib.orc_euler_flux.argtypes = [_d, _pd, _pd, _pd]
+    lib.orc_hllc.argtypes = [_d, _pd, _pd, _d, _d, _pd]
+    lib.orc_conserved.argtypes = [_d, _pd, _pd]
+    lib.orc_characteristic.argtypes = [_d, _pd, _pd, _d, _d, _pd, _pd]
+    lib.orc_simple_weno.argtypes = [_pd, _pd, C.c_int, C.c_int, _pd]
+    lib.orc_characteristic_weno.argtypes = [_d, _pd, _d, _d, _pd, _pd, C.c_int, C.c_int, _pd]
+    lib.orc_positivity.argtypes = [_d, _pd, C.c_int, _pd, C.c_int, _d, _d, C.POINTER(_d)]
+    lib.orc_modal_decay_troubled.argtypes = [_pd, C.c_int, _d, C.POINTER(C.c_int)]
     for name in ("orc_tri_topology", "orc_line_topology", "orc_tri_trace", "orc_stencil",
                  "orc_residual", "orc_limit", "orc_stable_dt", "orc_step", "orc_mass",
-                 "orc_project", "orc_run"):
+                 "orc_project", "orc_run", "orc_basis", "orc_rule", "orc_euler_flux",
+                 "orc_hllc", "orc_conserved", "orc_characteristic", "orc_simple_weno",
+                 "orc_characteristic_weno", "orc_positivity", "orc_modal_decay_troubled"):
         getattr(lib, name).restype = C.c_int
     _libs[kind] = lib
     return lib
@@ -82,6 +94,87 @@ def load(kind: str = "port") -> C.CDLL:
 
 class OracleError(RuntimeError):
     pass
+
+
+class Routines:
+    """Single reference routines of one backend (known-answer tests)."""
+
+    def __init__(self, kind: str = "port"):
+        self.lib = load(kind)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OracleError(self.lib.orc_error().decode())
+
+    @staticmethod
+    def _a(x):
+        return np.ascontiguousarray(x, np.float64).ravel()
+
+    def legendre(self, xi, p, gradient=False):
+        v, g, d = np.zeros(p + 1), np.zeros(p + 1), np.zeros(1)
+        self._check(self.lib.orc_basis(0, p, xi, 0.0, int(gradient), v, g, d))
+        return (v, g) if gradient else v
+
+    def dubiner(self, xi, eta, p, gradient=False):
+        n = (p + 1) * (p + 2) // 2
+        v, gx, ge = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._check(self.lib.orc_basis(1, p, xi, eta, int(gradient), v, gx, ge))
+        return (v, gx, ge) if gradient else v
+
+    def rule(self, kind, n, edge=0):
+        a, b, w = np.zeros(64), np.zeros(64), np.zeros(64)
+        count = self.lib.orc_rule(kind, n, edge, a, b, w)
+        if count < 0:
+            raise OracleError(self.lib.orc_error().decode())
+        return a[:count], b[:count], w[:count]
+
+    def euler_flux(self, gamma, state):
+        fx, fy = np.zeros(4), np.zeros(4)
+        self._check(self.lib.orc_euler_flux(gamma, self._a(state), fx, fy))
+        return fx, fy
+
+    def hllc(self, gamma, left, right, nx, ny):
+        f = np.zeros(4)
+        self._check(self.lib.orc_hllc(gamma, self._a(left), self._a(right), nx, ny, f))
+        return f
+
+    def conserved(self, gamma, prim):
+        s = np.zeros(4)
+        self._check(self.lib.orc_conserved(gamma, self._a(prim), s))
+        return s
+
+    def characteristic(self, gamma, average, change, nx, ny):
+        w, b = np.zeros(4), np.zeros(4)
+        self._check(self.lib.orc_characteristic(gamma, self._a(average), self._a(change), nx,
+                                                ny, w, b))
+        return w, b
+
+    def simple_weno(self, target, candidates, K, n):
+        out = np.zeros(n)
+        cand = self._a(candidates) if K else np.zeros(1)
+        self._check(self.lib.orc_simple_weno(self._a(target), cand, K, n, out))
+        return out
+
+    def characteristic_weno(self, gamma, average, nx, ny, target, candidates, K, n):
+        out = np.zeros(4 * n)
+        cand = self._a(candidates) if K else np.zeros(1)
+        self._check(self.lib.orc_characteristic_weno(gamma, self._a(average), nx, ny,
+                                                     self._a(target), cand, K, n, out))
+        return out
+
+    def positivity(self, gamma, coefficients, n, basis, density_floor, pressure_floor):
+        c = np.array(coefficients, np.float64).ravel()
+        b = self._a(basis)
+        s = _d()
+        self._check(self.lib.orc_positivity(gamma, c, n, b, b.size // n, density_floor,
+                                            pressure_floor, C.byref(s)))
+        return c, s.value
+
+    def modal_decay_troubled(self, coefficients, p, threshold):
+        t = C.c_int()
+        self._check(self.lib.orc_modal_decay_troubled(self._a(coefficients), p, threshold,
+                                                      C.byref(t)))
+        return bool(t.value)
 
 
 class Case:

@@ -128,6 +128,38 @@ int orc_run(orc_case *c, double cfl, int limiting, double threshold, double dens
             uint64_t max_steps, uint64_t history_interval, double *state, uint64_t *final_step,
             double *final_time, double *history, int64_t history_capacity, int64_t *history_rows);
 
+/* ---- function-level hooks (known-answer tests of single routines) ---- */
+
+/* Basis values: kind 0 Legendre (xi) with p+1 values, kind 1 Dubiner
+ * (xi, eta) with (p+1)(p+2)/2 values; gradient != 0 fills gx (and ge). */
+int orc_basis(int kind, int p, double xi, double eta, int gradient, double *values, double *gx,
+              double *ge);
+/* Rules: kind 0 Gauss-Legendre n points (x, w); kind 1 triangle volume rule
+ * of order n ((n+1)^2 points: xi, eta, w); kind 2 edge rule of order n for
+ * edge `edge` (xi, eta, w). Returns the point count (or -1). */
+int orc_rule(int kind, int n, int edge, double *a, double *b, double *w);
+/* Euler physics on conserved states [4]. */
+int orc_euler_flux(double gamma, const double *state, double *fx, double *fy);
+int orc_hllc(double gamma, const double *left, const double *right, double nx, double ny,
+             double *flux);
+int orc_conserved(double gamma, const double *primitive, double *state);
+/* Characteristic transform of change[4] frozen at average[4] and back. */
+int orc_characteristic(double gamma, const double *average, const double *change, double nx,
+                       double ny, double *waves, double *back);
+/* simple_weno on one field: target [n], candidates [K][n] -> limited [n]. */
+int orc_simple_weno(const double *target, const double *candidates, int K, int n,
+                    double *limited);
+/* euler_characteristic_weno: target [4][n], candidates [K][4][n]. */
+int orc_characteristic_weno(double gamma, const double *average, double nx, double ny,
+                            const double *target, const double *candidates, int K, int n,
+                            double *limited);
+/* limit_euler_positivity on packed [4][n] coefficients with check basis
+ * [points][n]; returns the total scale in *scale. */
+int orc_positivity(double gamma, double *coefficients, int n, const double *basis, int points,
+                   double density_floor, double pressure_floor, double *scale);
+/* modal_decay_troubled of one contiguous triangle polynomial of order p. */
+int orc_modal_decay_troubled(const double *coefficients, int p, double threshold, int *troubled);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,7 +41,11 @@
 #include "wasl/flux/hllc.hpp"
 #include "wasl/limiting/euler_limiter_stencil.hpp"
 #include "wasl/limiting/euler_modal_limiter.hpp"
+#include "wasl/limiting/positivity/euler_positivity.hpp"
 #include "wasl/limiting/troubled_cell/modal_decay.hpp"
+#include "wasl/limiting/weno/euler_characteristic.hpp"
+#include "wasl/limiting/weno/euler_characteristic_weno.hpp"
+#include "wasl/limiting/weno/simple_weno.hpp"
 #include "wasl/mesh/line/build_line_mesh.hpp"
 #include "wasl/mesh/msh/msh_document.hpp"
 #include "wasl/mesh/prepared/mesh_from_prepared.hpp"
@@ -514,6 +518,112 @@ int orc_run(orc_case *c, double cfl, int limiting, double threshold, double dens
     if (history_rows)
       *history_rows = rows;
   });
+}
+
+// ---- function-level hooks: direct calls into the reference routines ----
+
+int orc_basis(int kind, int p, double xi, double eta, int gradient, double *values, double *gx,
+              double *ge) {
+  return guarded([&] {
+    if (kind == 0) {
+      wasl::legendre_basis(xi, p, values);
+      if (gradient)
+        wasl::legendre_basis_gradient(xi, p, gx);
+    } else {
+      wasl::dubiner_basis(xi, eta, p, values);
+      if (gradient)
+        wasl::dubiner_basis_gradient(xi, eta, p, gx, ge);
+    }
+  });
+}
+
+int orc_rule(int kind, int n, int edge, double *a, double *b, double *w) {
+  int count = -1;
+  if (guarded([&] {
+        if (kind == 0) {
+          const auto r = wasl::gauss_legendre(n);
+          count = r.size();
+          std::copy(r.points.begin(), r.points.end(), a);
+          std::copy(r.weights.begin(), r.weights.end(), w);
+        } else if (kind == 1) {
+          const auto r = wasl::triangle_volume_rule(n);
+          count = r.size();
+          std::copy(r.xi.begin(), r.xi.end(), a);
+          std::copy(r.eta.begin(), r.eta.end(), b);
+          std::copy(r.weights.begin(), r.weights.end(), w);
+        } else {
+          const auto r = wasl::triangle_edge_rules(n).at(static_cast<std::size_t>(edge));
+          count = static_cast<int>(r.xi.size());
+          std::copy(r.xi.begin(), r.xi.end(), a);
+          std::copy(r.eta.begin(), r.eta.end(), b);
+          std::copy(r.weights.begin(), r.weights.end(), w);
+        }
+      }) != 0)
+    return -1;
+  return count;
+}
+
+namespace {
+wasl::EulerConserved cons4(const double *s) { return {s[0], s[1], s[2], s[3]}; }
+void store4(const wasl::EulerConserved &s, double *out) {
+  for (int v = 0; v < 4; ++v)
+    out[v] = wasl::euler_component(s, v);
+}
+}  // namespace
+
+int orc_euler_flux(double gamma, const double *s, double *fx, double *fy) {
+  return guarded([&] {
+    const wasl::Euler eq{gamma};
+    store4(wasl::euler_flux_x(eq, cons4(s)), fx);
+    store4(wasl::euler_flux_y(eq, cons4(s)), fy);
+  });
+}
+
+int orc_hllc(double gamma, const double *l, const double *r, double nx, double ny,
+             double *flux) {
+  return guarded(
+      [&] { store4(wasl::hllc_flux(wasl::Euler{gamma}, cons4(l), cons4(r), nx, ny), flux); });
+}
+
+int orc_conserved(double gamma, const double *w, double *s) {
+  return guarded([&] {
+    store4(wasl::conserved(wasl::Euler{gamma}, wasl::EulerPrimitive{w[0], w[1], w[2], w[3]}), s);
+  });
+}
+
+int orc_characteristic(double gamma, const double *avg, const double *d, double nx, double ny,
+                       double *waves, double *back) {
+  return guarded([&] {
+    const wasl::Euler eq{gamma};
+    const auto c = wasl::euler_to_characteristic(eq, cons4(avg), cons4(d), nx, ny);
+    std::copy(c.begin(), c.end(), waves);
+    store4(wasl::euler_from_characteristic(eq, cons4(avg), c, nx, ny), back);
+  });
+}
+
+int orc_simple_weno(const double *target, const double *candidates, int K, int n,
+                    double *limited) {
+  return guarded([&] { wasl::simple_weno(target, candidates, K, n, limited); });
+}
+
+int orc_characteristic_weno(double gamma, const double *avg, double nx, double ny,
+                            const double *target, const double *candidates, int K, int n,
+                            double *limited) {
+  return guarded([&] {
+    wasl::euler_characteristic_weno(wasl::Euler{gamma}, cons4(avg), nx, ny, target, candidates,
+                                    K, n, limited);
+  });
+}
+
+int orc_positivity(double gamma, double *c, int n, const double *basis, int points, double rf,
+                   double pf, double *scale) {
+  return guarded([&] {
+    *scale = wasl::limit_euler_positivity(wasl::Euler{gamma}, c, n, basis, points, rf, pf);
+  });
+}
+
+int orc_modal_decay_troubled(const double *c, int p, double threshold, int *troubled) {
+  return guarded([&] { *troubled = wasl::modal_decay_troubled(c, p, threshold) ? 1 : 0; });
 }
 
 }  // extern "C"

@@ -1,0 +1,100 @@
+"""Writes the golden fixtures under tests/golden/ (run in a container where
+the reference is present: `python tests/golden/make_fixtures.py`).
+
+  case_regression_rows.txt  the 14 on-path rows (euler-d2-*, scalar-d1-*) of
+                            the reference's stored case-regression metrics
+                            (cpp/tests/runtime/case_regression_reference.txt),
+                            copied verbatim as data;
+  reference_vectors.npz     inputs and outputs of the reference's own code
+                            (oracle/_ref, compiled from the reference sources)
+                            on small seeded meshes: residuals P0-P3, limiter
+                            flags and states, stepping trajectories, 1D
+                            advection.
+
+The reference cannot travel to the GPU box; these vectors do.
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+REFERENCE_TXT = "/root/reference/cpp/tests/runtime/case_regression_reference.txt"
+
+
+def regression_rows() -> None:
+    keep = []
+    with open(REFERENCE_TXT) as fh:
+        for line in fh:
+            if line.startswith("#") or line.startswith("euler-d2-") or line.startswith(
+                    "scalar-d1-"):
+                keep.append(line)
+    with open(os.path.join(HERE, "case_regression_rows.txt"), "w") as fh:
+        fh.writelines(keep)
+
+
+def vectors() -> None:
+    import cases as K
+    from oracle import oracle as O
+    from wasl_amd import meshgen as MG
+
+    out = {}
+    m = K.vortex_mesh(8)
+    for p in range(4):
+        c = K.oracle_tri(m, p, kind="reference")
+        U = c.project(O.IC_VORTEX, K.VORTEX)
+        rng = np.random.default_rng(100 + p)
+        U = U * (1.0 + 1e-3 * rng.standard_normal(U.size))
+        out[f"vortex_p{p}_state"] = U
+        out[f"vortex_p{p}_residual"] = c.residual(U)
+        out[f"vortex_p{p}_dt"] = np.array([c.stable_dt(U, 0.15)])
+    c = K.oracle_tri(m, 2, kind="reference")
+    U = c.project(O.IC_VORTEX, K.VORTEX)
+    traj = [U]
+    for _ in range(3):
+        U = c.step(U, c.stable_dt(U, 0.15))
+        traj.append(U)
+    out["vortex_p2_steps"] = np.stack(traj)
+
+    r = K.rarefaction_mesh(8)
+    for p in (1, 2, 3):
+        c = K.oracle_tri(r, p, boundary=O.OUTFLOW, kind="reference")
+        U = c.project(O.IC_EULER_REGIONS, K.JUMP)
+        Ul, flags = c.limit(U, 1e-3, 1e-13, 1e-13)
+        out[f"jump_p{p}_state"] = U
+        out[f"jump_p{p}_limited"] = Ul
+        out[f"jump_p{p}_flags"] = flags
+        out[f"jump_p{p}_smooth_residual"] = c.residual(c.project(O.IC_SMOOTH, K.SMOOTH))
+    c = K.oracle_tri(r, 2, boundary=O.OUTFLOW, kind="reference")
+    U = c.project(O.IC_RAREFACTION, K.RAREFACTION)
+    U, _ = c.limit(U, 1e-3, 1e-13, 1e-13)
+    traj = [U]
+    for _ in range(3):
+        U = c.step(U, c.stable_dt(U, 0.1), True, 1e-3, 1e-13, 1e-13)
+        traj.append(U)
+    out["rarefaction_p2_limited_steps"] = np.stack(traj)
+
+    line = MG.jittered_line(200)
+    c = O.line_scalar(line.node_tags, line.node_x, line.segments, 2, 1.0, kind="reference")
+    U = c.project(O.IC_SCALAR_PULSE)
+    dt = c.stable_dt(U, 0.1)
+    traj = [U]
+    for _ in range(10):
+        U = c.step(U, dt)
+        traj.append(U)
+    out["pulse_p2_steps"] = np.stack(traj)
+    out["pulse_p2_dt"] = np.array([dt])
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
+
+
+if __name__ == "__main__":
+    regression_rows()
+    vectors()
+    print("fixtures written to", HERE)

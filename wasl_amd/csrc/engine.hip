@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -139,7 +140,13 @@ struct wasl_hip_engine {
   int64_t last_stage_launches = 0;
   bool timing = false;
 
+  // captured batch of steps for wasl_hip_advance
+  hipGraphExec_t graph_exec = nullptr;
+  bool use_graphs = true;
+
   ~wasl_hip_engine() {
+    if (graph_exec)
+      (void)hipGraphExecDestroy(graph_exec);
     for (auto e : ev)
       (void)hipEventDestroy(e);
     if (ctl_host)
@@ -449,6 +456,8 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     throw std::runtime_error("engine: only the FP64 precision is available in this build");
   g.mesh = mesh;
   HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  if (const char *env = std::getenv("WASL_HIP_GRAPHS"))
+    g.use_graphs = std::atoi(env) != 0;
   HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.ctl_host), sizeof(whip::RunCtl)));
   HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.result_host), sizeof(double)));
   size_t &tot = g.device_bytes;
@@ -1053,15 +1062,37 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     HIPCHECK(hipEventCreate(&t1));
     HIPCHECK(hipEventRecord(t0, g->stream));
     while (true) {
-      uint64_t batch = kBatch;
+      // Steps beyond max_steps would only be skipped on the device; do not
+      // launch them.
+      if (cur_step >= max_steps)
+        break;
+      uint64_t batch = std::min<uint64_t>(kBatch, max_steps - cur_step);
       if (history) {
         const uint64_t to_next = history_interval - (cur_step % history_interval);
         batch = std::min<uint64_t>(batch, to_next);
       }
       int idx = 0;
-      for (uint64_t i = 0; i < batch; ++i) {
-        launch_step(*g, 0.0, dctl, parity, timed, idx);
-        parity ^= 1;
+      if (!timed && batch == kBatch && parity == 0 && g->use_graphs) {
+        // A full batch of an even number of steps returns to parity 0, so one
+        // captured graph serves every full batch.
+        if (!g->graph_exec) {
+          hipGraph_t graph;
+          HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+          int par = 0, unused = 0;
+          for (int i = 0; i < kBatch; ++i) {
+            launch_step(*g, 0.0, dctl, par, false, unused);
+            par ^= 1;
+          }
+          HIPCHECK(hipStreamEndCapture(g->stream, &graph));
+          HIPCHECK(hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0));
+          HIPCHECK(hipGraphDestroy(graph));
+        }
+        HIPCHECK(hipGraphLaunch(g->graph_exec, g->stream));
+      } else {
+        for (uint64_t i = 0; i < batch; ++i) {
+          launch_step(*g, 0.0, dctl, parity, timed, idx);
+          parity ^= 1;
+        }
       }
       HIPCHECK(hipMemcpyAsync(h, g->ctl.p, sizeof(*h), hipMemcpyDeviceToHost, g->stream));
       HIPCHECK(hipStreamSynchronize(g->stream));
