@@ -104,8 +104,13 @@ def main(argv=None) -> int:
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--cells", type=int, default=128, help="quads per axis (2 n^2 triangles)")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--stage-kernel", type=int, default=0,
+                    help="0 autotuned, 1 point split, 2 variable split")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a plain copy kernel of known "
+                         "size, to scale hardware byte counters)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -122,13 +127,24 @@ def main(argv=None) -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
 
+    if args.calibrate_copy:
+        n = args.calibrate_copy * 2 ** 20 // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
+
     p = args.order
     mesh_in = MG.jittered_triangles(args.cells, (0.0, 0.0), (10.0, 10.0), 0.15)
     mesh = wasl_amd.Mesh.from_input(mesh_in)
-    eng = wasl_amd.Engine(mesh, wasl_amd.EULER, p, lanes=args.lanes)
+    eng = wasl_amd.Engine(mesh, wasl_amd.EULER, p, lanes=args.lanes,
+                          stage_kernel=args.stage_kernel)
     eng.project(wasl_amd.IC_VORTEX, (5.0, 5.0, 5.0))
     E, Np, V = eng.E, eng.Np, eng.V
-    log(f"engine: {E} triangles, P{p}, lanes/element {eng.lanes}, "
+    log(f"engine: {E} triangles, P{p}, stage kernel {eng.stage_kernel}, "
+        f"lanes/element {eng.lanes}, "
         f"{eng.device_bytes / 2**20:.1f} MiB on device")
 
     cfl = 0.15
@@ -182,6 +198,7 @@ def main(argv=None) -> int:
                         f"HLLC, SSP-RK3, CFL {cfl}, device dt, plain FP64",
             "elements": E, "modes": Np, "variables": V,
             "dofs": E * Np * V, "lanes_per_element": eng.lanes,
+            "stage_kernel": ["point split", "variable split"][eng.stage_kernel - 1],
             "stage_rate": 3 * args.steps / wall, "state_finite": finite,
         },
         "roofline": {
@@ -191,7 +208,7 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "whip::euler_stage",
+            "kernel": ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1],
             "stage_kernel_us": stage_ms * 1e3,
             "algorithmic_bytes_per_launch": bytes_stage,
             "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,

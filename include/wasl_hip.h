@@ -140,6 +140,9 @@ typedef struct {
   double pressure_floor;     /* default 1e-10 */
   int32_t precision;         /* WASL_HIP_FP64, WASL_HIP_MIXED or WASL_HIP_BF16 */
   int32_t lanes_per_element; /* 0: autotuned tile shape */
+  int32_t stage_kernel;      /* 0: autotuned; 1: point split (1, 2 or 4 lanes per element,
+                                each lane a share of the quadrature points); 2: variable split
+                                (4 or 8 lanes per element, each lane one conserved variable) */
 } wasl_hip_numerics;
 
 /* Set fields to the reference defaults (config.rs:85-103). */

@@ -53,8 +53,8 @@ def oracle_tri(m, order, boundary=O.PERIODIC, kind="port"):
 
 
 def engine_tri(wasl, m, order, boundary=0, limiting=False, threshold=1e-3, floors=1e-10,
-               lanes=0, precision=0):
+               lanes=0, precision=0, stage_kernel=0):
     mesh = wasl.Mesh.from_input(m)
     return wasl.Engine(mesh, wasl.EULER, order, boundary=(boundary, boundary), limiting=limiting,
                        sensor_threshold=threshold, density_floor=floors, pressure_floor=floors,
-                       lanes=lanes, precision=precision)
+                       lanes=lanes, precision=precision, stage_kernel=stage_kernel)

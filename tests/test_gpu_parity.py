@@ -17,25 +17,29 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4])
+TILES = [(1, 1), (1, 2), (1, 4), (2, 4), (2, 8)]  # (stage kernel, lanes per element)
+
+
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("order", [0, 1, 2, 3])
-def test_euler_residual_periodic(gpu, order, lanes):
+def test_euler_residual_periodic(gpu, order, tile):
     m = K.vortex_mesh(10)
     o = K.oracle_tri(m, order)
-    g = K.engine_tri(gpu, m, order, lanes=lanes)
+    g = K.engine_tri(gpu, m, order, stage_kernel=tile[0], lanes=tile[1])
     U = o.project(O.IC_VORTEX, K.VORTEX)
     rng = np.random.default_rng(order)
     U = U * (1.0 + 1e-3 * rng.standard_normal(U.size))
     err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
-    print(f"P{order} lanes={lanes} residual rel L-inf {err:.3e}")
+    print(f"P{order} tile={tile} residual rel L-inf {err:.3e}")
     assert err <= K.STEP_TOL
 
 
+@pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_euler_residual_transmissive(gpu, order):
+def test_euler_residual_transmissive(gpu, order, tile):
     m = K.rarefaction_mesh(10)
     o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
-    g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW)
+    g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW, stage_kernel=tile[0], lanes=tile[1])
     U = o.project(O.IC_SMOOTH, K.SMOOTH)
     err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
     print(f"P{order} transmissive residual rel L-inf {err:.3e}")
@@ -65,11 +69,12 @@ def test_stable_dt(gpu, order):
     assert abs(g.stable_time_step(0.15) - o.stable_dt(U, 0.15)) <= 1e-15 * o.stable_dt(U, 0.15)
 
 
+@pytest.mark.parametrize("tile", [(1, 4), (2, 4), (2, 8)])
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_step_unlimited(gpu, order):
+def test_step_unlimited(gpu, order, tile):
     m = K.vortex_mesh(10)
     o = K.oracle_tri(m, order)
-    g = K.engine_tri(gpu, m, order)
+    g = K.engine_tri(gpu, m, order, stage_kernel=tile[0], lanes=tile[1])
     U = o.project(O.IC_VORTEX, K.VORTEX)
     dt = o.stable_dt(U, 0.15)
     g.state = U

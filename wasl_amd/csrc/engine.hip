@@ -99,12 +99,13 @@ void upload(Buffer &b, const std::vector<T> &v, size_t &total, hipStream_t s) {
     HIPCHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
-int choose_lanes(int64_t elements) {
-  if (elements < 131072)
-    return 4;
-  if (elements < 524288)
-    return 2;
-  return 1;
+// Default tile shape of the Euler stage kernel when the caller names none.
+void choose_tile(int order, int &kernel, int &lanes) {
+  // Measured on MI355X (32k and 524k triangles): the point split with four
+  // lanes per element is fastest up to P2, the variable split with four lanes
+  // from P3 on (its per-lane register state grows with one variable, not four).
+  kernel = order >= 3 ? 2 : 1;
+  lanes = 4;
 }
 
 }  // namespace
@@ -114,6 +115,7 @@ struct wasl_hip_engine {
   int dim = 2, p = 0, Np = 1, V = 4;
   int E = 0, F = 0, NB = 0;
   int lanes = 1;
+  int stage_kernel = 1;       // 1 point split, 2 variable split
   double hmin = 0.0;
   hipStream_t stream = nullptr;
   size_t device_bytes = 0;
@@ -126,7 +128,7 @@ struct wasl_hip_engine {
   // state and stage buffers
   Buffer u, w1, w2;
   // geometry and tables
-  Buffer geo, nbr, meta, fn, bdonor, bbasis, table, inv_jac;
+  Buffer geo, nbr, meta, fn, bdonor, bbasis, table, vtable, inv_jac;
   // limiter
   Buffer lnbr, T, nrm, check, flags, list, counters, side, err;
   // control and reductions
@@ -169,6 +171,7 @@ struct wasl_hip_engine {
     m.bdonor = bdonor.as<int>();
     m.bbasis = bbasis.as<double>();
     m.table = table.as<double>();
+    m.vtable = vtable.as<double>();
     m.gamma = num.heat_capacity_ratio;
     return m;
   }
@@ -247,6 +250,26 @@ std::vector<double> euler_tables(int p) {
   return t;
 }
 
+// Volume tables padded to a multiple of 8 points with zero weights (the
+// padded points reuse point 0's basis values so every state stays finite).
+std::vector<double> euler_vtables(int p) {
+  const int Np = whip::tri_modes(p), NQ = (p + 3) * (p + 3), NQ8 = (NQ + 7) / 8 * 8;
+  std::vector<double> t(3 * NQ8 * Np, 0.0);
+  const whip::TriRule vol = whip::triangle_rule(p + 2);
+  std::vector<double> b(Np), gx(Np), ge(Np);
+  for (int q = 0; q < NQ8; ++q) {
+    const int src = q < NQ ? q : 0;
+    whip::dubiner(vol.xi[src], vol.eta[src], p, b.data());
+    whip::dubiner_gradient(vol.xi[src], vol.eta[src], p, gx.data(), ge.data());
+    for (int m = 0; m < Np; ++m) {
+      t[q * Np + m] = b[m];
+      t[NQ8 * Np + q * Np + m] = q < NQ ? vol.w[q] * gx[m] : 0.0;
+      t[2 * NQ8 * Np + q * Np + m] = q < NQ ? vol.w[q] * ge[m] : 0.0;
+    }
+  }
+  return t;
+}
+
 std::vector<double> line_tables(int p) {
   const int Np = p + 1, NQ = p > 0 ? p : 1;
   std::vector<double> t(2 * NQ * Np + 2 * Np, 0.0);
@@ -270,8 +293,15 @@ std::vector<double> line_tables(int p) {
 template <int P>
 void launch_euler_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
   const whip::EulerMesh m = g.euler_mesh();
-  const int per = 256 / g.lanes;
+  const int per = g.stage_kernel == 2 ? 256 / g.lanes : 256 / g.lanes;
   const dim3 grid((g.E + per - 1) / per);
+  if (g.stage_kernel == 2) {
+    if (g.lanes == 8)
+      hipLaunchKernelGGL((whip::euler_stage_vs<P, 2>), grid, dim3(256), 0, g.stream, m, a);
+    else
+      hipLaunchKernelGGL((whip::euler_stage_vs<P, 1>), grid, dim3(256), 0, g.stream, m, a);
+    return;
+  }
   switch (g.lanes) {
   case 1:
     hipLaunchKernelGGL((whip::euler_stage<P, 1>), grid, dim3(256), 0, g.stream, m, a);
@@ -472,9 +502,17 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     g.Np = whip::tri_modes(g.p);
     g.E = static_cast<int>(mesh.tris.size());
     g.F = static_cast<int>(mesh.faces.size());
-    g.lanes = num.lanes_per_element > 0 ? num.lanes_per_element : choose_lanes(g.E);
-    if (g.lanes != 1 && g.lanes != 2 && g.lanes != 4)
-      throw std::runtime_error("engine: lanes_per_element must be 1, 2 or 4");
+    choose_tile(g.p, g.stage_kernel, g.lanes);
+    if (num.stage_kernel != 0 || num.lanes_per_element != 0) {
+      g.stage_kernel = num.stage_kernel != 0 ? num.stage_kernel : (num.lanes_per_element > 4 ? 2 : 1);
+      g.lanes = num.lanes_per_element != 0 ? num.lanes_per_element : (g.stage_kernel == 2 ? 4 : 4);
+    }
+    if (g.stage_kernel == 1 && g.lanes != 1 && g.lanes != 2 && g.lanes != 4)
+      throw std::runtime_error("engine: the point-split kernel takes 1, 2 or 4 lanes per element");
+    if (g.stage_kernel == 2 && g.lanes != 4 && g.lanes != 8)
+      throw std::runtime_error("engine: the variable-split kernel takes 4 or 8 lanes per element");
+    if (g.stage_kernel != 1 && g.stage_kernel != 2)
+      throw std::runtime_error("engine: stage_kernel must be 0, 1 or 2");
     const int NE = g.p + 1;
     const size_t E = static_cast<size_t>(g.E);
 
@@ -540,6 +578,16 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     upload(g.bdonor, bdon, tot, g.stream);
     upload(g.bbasis, bbas, tot, g.stream);
     upload(g.table, euler_tables(g.p), tot, g.stream);
+    {
+      // Reference-element tables of order p into the constant slot of that
+      // order; every engine of the same order writes the same values.
+      const std::vector<double> vt = euler_vtables(g.p);
+      HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(whip::g_vtable), vt.data(),
+                                      vt.size() * sizeof(double),
+                                      static_cast<size_t>(g.p) * whip::kVTableMax * sizeof(double),
+                                      hipMemcpyHostToDevice, g.stream));
+      HIPCHECK(hipStreamSynchronize(g.stream));
+    }
     g.hmin = whip::minimum_length(mesh);
 
     if (num.limiting && g.p > 0) {
@@ -867,6 +915,7 @@ int64_t wasl_hip_engine_size(const wasl_hip_engine *g, int what) {
   case 4: return g->F;
   case 5: return static_cast<int64_t>(g->device_bytes);
   case 6: return g->lanes;
+  case 8: return g->stage_kernel;
   case 7:
     const_cast<wasl_hip_engine *>(g)->build_stencil_host();
     return g->stencil.checks();

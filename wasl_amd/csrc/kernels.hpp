@@ -33,6 +33,16 @@ template <int P> struct Tri {
   static constexpr int kBr = kBl + 3 * NE * Np;          // [3][3][2][NE][Np]
   static constexpr int kWe = kBr + 18 * NE * Np;         // [3][NE]
   static constexpr int kTable = kWe + 3 * NE;
+  // Volume tables padded to a multiple of 8 points (zero weights) for the
+  // variable-split kernel: B [NQ8][Np], W_xi [NQ8][Np], W_eta [NQ8][Np].
+  static constexpr int NQ8 = (NQ + 7) / 8 * 8;
+  static constexpr int kVB = 0;
+  static constexpr int kVWx = NQ8 * Np;
+  static constexpr int kVWy = 2 * NQ8 * Np;
+  static constexpr int kVTable = 3 * NQ8 * Np;
+  // Edge tables alone (the tail of the full table), staged in LDS.
+  static constexpr int kEdge = kBl;
+  static constexpr int kEdgeTable = kTable - kBl;
 };
 
 template <int P> struct Line {
@@ -92,6 +102,7 @@ struct EulerMesh {
   const int *bdonor;          // [NB][NE] periodic donors (or null: transmissive)
   const double *bbasis;       // [NB][NE][Np]
   const double *table;        // Tri<P>::kTable doubles
+  const double *vtable;       // Tri<P>::kVTable doubles (padded volume tables)
   double gamma;
 };
 
@@ -152,50 +163,58 @@ struct C4 {
   double r, mx, my, E;
 };
 
-__device__ __forceinline__ void normal_flux(double g1, const C4 &s, double nx, double ny,
-                                            double &u, double &v, double &p, C4 &f) {
-  const double inv = 1.0 / s.r;
-  u = s.mx * inv;
-  v = s.my * inv;
-  p = g1 * (s.E - 0.5 * (s.mx * u + s.my * v));
-  const double un = u * nx + v * ny;
-  f.r = s.r * un;
-  f.mx = s.mx * un + p * nx;
-  f.my = s.my * un + p * ny;
-  f.E = (s.E + p) * un;
+// Reciprocal to within an ulp: the hardware estimate refined by two Newton
+// steps (5 instructions instead of the ~10 of a correctly rounded divide).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+struct Side {
+  double inv_r, u, v, p, un, c;
+};
+
+__device__ __forceinline__ Side side_of(double gamma, const C4 &s, double nx, double ny) {
+  Side o;
+  o.inv_r = rcp(s.r);
+  o.u = s.mx * o.inv_r;
+  o.v = s.my * o.inv_r;
+  o.p = (gamma - 1.0) * (s.E - 0.5 * (s.mx * o.u + s.my * o.v));
+  o.un = o.u * nx + o.v * ny;
+  o.c = sqrt(gamma * o.p * o.inv_r);
+  return o;
+}
+
+__device__ __forceinline__ C4 normal_flux(const C4 &s, const Side &w, double nx, double ny) {
+  return {s.r * w.un, s.mx * w.un + w.p * nx, s.my * w.un + w.p * ny, (s.E + w.p) * w.un};
 }
 
 // HLLC flux with Davis wave-speed bounds, evaluated (left, right, n_left)
 // (flux/hllc.hpp:40-74).
 __device__ inline C4 hllc(double gamma, const C4 &L, const C4 &R, double nx, double ny) {
-  const double g1 = gamma - 1.0;
-  double ul, vl, pl, ur, vr, pr;
-  C4 fl, fr;
-  normal_flux(g1, L, nx, ny, ul, vl, pl, fl);
-  normal_flux(g1, R, nx, ny, ur, vr, pr, fr);
-  const double unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
-  const double cl = sqrt(gamma * pl / L.r), cr = sqrt(gamma * pr / R.r);
-  const double sl = fmin(unl - cl, unr - cr);
-  const double sr = fmax(unl + cl, unr + cr);
+  const Side wl = side_of(gamma, L, nx, ny), wr = side_of(gamma, R, nx, ny);
+  const double sl = fmin(wl.un - wl.c, wr.un - wr.c);
+  const double sr = fmax(wl.un + wl.c, wr.un + wr.c);
   if (sl >= 0.0)
-    return fl;
+    return normal_flux(L, wl, nx, ny);
   if (sr <= 0.0)
-    return fr;
-  const double sm = (pr - pl + L.r * unl * (sl - unl) - R.r * unr * (sr - unr)) /
-                    (L.r * (sl - unl) - R.r * (sr - unr));
+    return normal_flux(R, wr, nx, ny);
+  const double sm = (wr.p - wl.p + L.r * wl.un * (sl - wl.un) - R.r * wr.un * (sr - wr.un)) *
+                    rcp(L.r * (sl - wl.un) - R.r * (sr - wr.un));
   const bool left = sm >= 0.0;
   const C4 &S = left ? L : R;
-  const C4 &F = left ? fl : fr;
+  const Side &w = left ? wl : wr;
   const double s = left ? sl : sr;
-  const double un = left ? unl : unr;
-  const double uu = left ? ul : ur, vv = left ? vl : vr, pp = left ? pl : pr;
-  const double rs = S.r * (s - un) / (s - sm);
-  const double shift = sm - un;
-  const double es = S.E / S.r + shift * (sm + pp / (S.r * (s - un)));
+  const C4 F = normal_flux(S, w, nx, ny);
+  const double k = s - w.un;
+  const double rs = S.r * k * rcp(s - sm);
+  const double shift = sm - w.un;
+  const double es = S.E * w.inv_r + shift * (sm + w.p * w.inv_r * rcp(k));
   C4 o;
   o.r = F.r + s * (rs - S.r);
-  o.mx = F.mx + s * (rs * (uu + shift * nx) - S.mx);
-  o.my = F.my + s * (rs * (vv + shift * ny) - S.my);
+  o.mx = F.mx + s * (rs * (w.u + shift * nx) - S.mx);
+  o.my = F.my + s * (rs * (w.v + shift * ny) - S.my);
   o.E = F.E + s * (rs * es - S.E);
   return o;
 }
@@ -269,8 +288,35 @@ __device__ inline void stage_epilogue(RunCtl *ctl, int parity, int stage, double
 
 // ------------------------------------------------------ 2D Euler stage ----
 
+// Trace of one element's coefficients c[4][Np] against basis values b[Np].
+template <int Np> __device__ __forceinline__ C4 trace_of(const double (&c)[4][Np],
+                                                         const double *b) {
+  double t[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double x = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      x += c[v][m] * b[m];
+    t[v] = x;
+  }
+  return {t[0], t[1], t[2], t[3]};
+}
+
+template <int Np>
+__device__ __forceinline__ void load_coefficients(const double *U, int E, int e,
+                                                  double (&c)[4][Np]) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      c[v][m] = U[(v * Np + m) * E + e];
+}
+
+// Two co-resident waves per SIMD: the register budget is capped at 256.
 template <int P, int LANES>
-__global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void euler_stage(EulerMesh mesh, StageArgs a) {
   using D = Tri<P>;
   constexpr int Np = D::Np, NQ = D::NQ, NE = D::NE;
   constexpr int PER_BLOCK = 256 / LANES;
@@ -284,10 +330,6 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
   if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
-  for (int i = threadIdx.x; i < D::kTable; i += 256)
-    tab[i] = mesh.table[i];
-  __syncthreads();
-
   const int E = mesh.E;
   const int sub = threadIdx.x % LANES;
   const int e_raw = blockIdx.x * PER_BLOCK + threadIdx.x / LANES;
@@ -295,12 +337,24 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
 
+  // Global requests first: own coefficients and, with one edge per lane, the
+  // edge record and the neighbour's coefficients, so their latency overlaps
+  // the table staging and the volume integral.
   double c[4][Np];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
-      c[v][m] = a.uin[(v * Np + m) * E + e];
+  load_coefficients<Np>(a.uin, E, e, c);
+  constexpr bool kPrefetch = LANES == 4 && P <= 2;
+  double nc[4][kPrefetch ? Np : 1];
+  int meta0 = 0, nb0 = -1;
+  if constexpr (kPrefetch) {
+    const int k0 = sub < 3 ? sub : 0;
+    meta0 = mesh.meta[k0 * E + e];
+    nb0 = mesh.nbr[k0 * E + e];
+    load_coefficients<Np>(a.uin, E, nb0 >= 0 ? nb0 : e, nc);
+  }
+
+  for (int i = threadIdx.x; i < D::kTable; i += 256)
+    tab[i] = mesh.table[i];
+  __syncthreads();
 
   double acc[4][Np];
 #pragma unroll
@@ -314,24 +368,16 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
   if constexpr (P > 0) {
     const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
                  gD = mesh.geo[3 * E + e];
+#pragma unroll 1
     for (int it = 0; it < (NQ + LANES - 1) / LANES; ++it) {
       const int q = it * LANES + sub;
       if (q < NQ) {
-        const double *phi = tab + D::kBvol + q * Np;
-        double s[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double t = 0.0;
-#pragma unroll
-          for (int m = 0; m < Np; ++m)
-            t += c[v][m] * phi[m];
-          s[v] = t;
-        }
-        const double inv = 1.0 / s[0];
-        const double u = s[1] * inv, w = s[2] * inv;
-        const double p = g1 * (s[3] - 0.5 * (s[1] * u + s[2] * w));
-        const double fx[4] = {s[1], s[1] * u + p, s[2] * u, (s[3] + p) * u};
-        const double fy[4] = {s[2], s[1] * w, s[2] * w + p, (s[3] + p) * w};
+        const C4 s = trace_of<Np>(c, tab + D::kBvol + q * Np);
+        const double inv = rcp(s.r);
+        const double u = s.mx * inv, w = s.my * inv;
+        const double p = g1 * (s.E - 0.5 * (s.mx * u + s.my * w));
+        const double fx[4] = {s.mx, s.mx * u + p, s.my * u, (s.E + p) * u};
+        const double fy[4] = {s.my, s.mx * w, s.my * w + p, (s.E + p) * w};
         const double *wx = tab + D::kWx + q * Np;
         const double *wy = tab + D::kWy + q * Np;
 #pragma unroll
@@ -347,11 +393,18 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
   }
 
   // Faces: each lane takes whole edges; traces on both sides use the left
-  // side's quadrature point so the two elements of a face see identical
-  // flux bits (exact conservation).
+  // side's quadrature point and HLLC is always evaluated (left, right,
+  // n_left), so the two elements of a face see identical flux bits (exact
+  // conservation).
   for (int k = sub; k < 3; k += LANES) {
-    const int meta = mesh.meta[k * E + e];
-    const int nb = mesh.nbr[k * E + e];
+    int meta, nb;
+    if constexpr (kPrefetch) {
+      meta = meta0;
+      nb = nb0;
+    } else {
+      meta = mesh.meta[k * E + e];
+      nb = mesh.nbr[k * E + e];
+    }
     const double nx = mesh.fn[(3 * k) * E + e];
     const double ny = mesh.fn[(3 * k + 1) * E + e];
     const double scale = mesh.fn[(3 * k + 2) * E + e];
@@ -362,49 +415,58 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
     const double *br = tab + D::kBr + ((kl * 3 + kr) * 2 + fl) * NE * Np;
     const double *own = is_left ? bl : br;
     const double *oth = is_left ? br : bl;
+    // Neighbour traces at all edge points, streaming its coefficients once.
+    double so[NE][4];
+#pragma unroll
+    for (int q = 0; q < NE; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        so[q][v] = 0.0;
+    if (!boundary) {
+      const int src = nb;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int m = 0; m < Np; ++m) {
+          double x;
+          if constexpr (kPrefetch)
+            x = nc[v][m];
+          else
+            x = a.uin[(v * Np + m) * E + src];
+#pragma unroll
+          for (int q = 0; q < NE; ++q)
+            so[q][v] += x * oth[q * Np + m];
+        }
+    }
+#pragma unroll
     for (int q = 0; q < NE; ++q) {
-      C4 si, so;
-      double t[4];
+      const C4 si = trace_of<Np>(c, own + q * Np);
+      C4 so4 = {so[q][0], so[q][1], so[q][2], so[q][3]};
+      if (boundary) {
+        if (mesh.bdonor) {
+          const int slot = meta_slot(meta);
+          const int d = mesh.bdonor[slot * NE + q];
+          const double *db = mesh.bbasis + (slot * NE + q) * Np;
+          double t[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double x = 0.0;
+          for (int v = 0; v < 4; ++v) {
+            double x = 0.0;
 #pragma unroll
-        for (int m = 0; m < Np; ++m)
-          x += c[v][m] * own[q * Np + m];
-        t[v] = x;
-      }
-      si = {t[0], t[1], t[2], t[3]};
-      if (!boundary) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double x = 0.0;
-#pragma unroll
-          for (int m = 0; m < Np; ++m)
-            x += a.uin[(v * Np + m) * E + nb] * oth[q * Np + m];
-          t[v] = x;
+            for (int m = 0; m < Np; ++m)
+              x += a.uin[(v * Np + m) * E + d] * db[m];
+            t[v] = x;
+          }
+          so4 = {t[0], t[1], t[2], t[3]};
+        } else {
+          so4 = si;  // transmissive: exterior = own trace
         }
-        so = {t[0], t[1], t[2], t[3]};
-      } else if (mesh.bdonor) {
-        const int slot = meta_slot(meta);
-        const int d = mesh.bdonor[slot * NE + q];
-        const double *db = mesh.bbasis + (slot * NE + q) * Np;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double x = 0.0;
-#pragma unroll
-          for (int m = 0; m < Np; ++m)
-            x += a.uin[(v * Np + m) * E + d] * db[m];
-          t[v] = x;
-        }
-        so = {t[0], t[1], t[2], t[3]};
-      } else {
-        so = si;  // transmissive: exterior = own trace
       }
-      const C4 f = is_left ? hllc(g, si, so, nx, ny) : hllc(g, so, si, nx, ny);
+      const C4 L = is_left ? si : so4;
+      const C4 R = is_left ? so4 : si;
+      const C4 f = hllc(g, L, R, nx, ny);
       const double wq = tab[D::kWe + kl * NE + q] * scale;
-      const double sgn = is_left ? -1.0 : 1.0;
-      const double val[4] = {sgn * (f.r * wq), sgn * (f.mx * wq), sgn * (f.my * wq),
-                             sgn * (f.E * wq)};
+      const double sw = is_left ? -wq : wq;
+      const double val[4] = {f.r * sw, f.mx * sw, f.my * sw, f.E * sw};
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -499,6 +561,268 @@ __global__ __launch_bounds__(256) void euler_stage(EulerMesh mesh, StageArgs a) 
       if (ctl)
         atomicMax(&ctl->s[a.parity ^ 1].lambda,
                   static_cast<unsigned long long>(__double_as_longlong(mx)));
+    }
+  }
+  stage_epilogue(ctl, a.parity, a.stage, dt);
+}
+
+// ------------------------------------------- 2D Euler, variable split ----
+
+// Padded volume tables per order in the constant address space: the values
+// depend only on the reference element, and wave-uniform reads from here are
+// served by the scalar cache into SGPR operands.
+constexpr int kVTableMax = 3 * 40 * 10;   // Tri<3>::kVTable
+__constant__ double g_vtable[4][kVTableMax];
+
+// Orders LDS traffic between the lanes of one wave (all lanes of an element
+// live in the same wave, so no block barrier is needed).
+__device__ __forceinline__ void lane_exchange_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The same fused stage with a different work split: each element is served
+// by H quads of lanes; lane v of a quad owns conserved variable v -- its Np
+// coefficients and Np right-hand-side accumulators -- and the quads of an
+// element take interleaved groups of four volume points and whole edges.
+// States at points and point fluxes move between the four lanes of a quad
+// through a small LDS scratch, so no lane holds another variable's modes and
+// no cross-lane reduction of the accumulators is needed (a single one across
+// the H quads when H = 2). Volume tables are wave-uniform and read through
+// the scalar cache; edge tables depend on each element's face configuration
+// and are staged in LDS.
+template <int P, int H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void euler_stage_vs(EulerMesh mesh, StageArgs a) {
+  using D = Tri<P>;
+  constexpr int Np = D::Np, NE = D::NE;
+  constexpr int PER_BLOCK = 64 / H;
+  constexpr int NIT = (D::NQ + 4 * H - 1) / (4 * H);
+  __shared__ double tab[D::kEdgeTable];
+  __shared__ double xs[64][2][4][4];   // per quad: [traces or states][point][variable]
+  __shared__ double xf[64][4][8];      // per quad: [point][F_xi 0..3, F_eta 0..3] or [point][F 0..3]
+  __shared__ double red[4];
+
+  RunCtl *ctl = a.ctl;
+  double dt = a.dt;
+  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+    return;
+  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
+
+  const int E = mesh.E;
+  const int v = threadIdx.x & 3;              // owned variable
+  const int quad = threadIdx.x >> 2;          // quad slot in the block
+  const int h = quad % H;                     // quad index within the element
+  const int e_raw = blockIdx.x * PER_BLOCK + quad / H;
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+  const double g = mesh.gamma, g1 = g - 1.0;
+
+  double c[Np], acc[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    c[m] = a.uin[(v * Np + m) * E + e];
+    acc[m] = 0.0;
+  }
+  const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
+               gD = mesh.geo[3 * E + e];
+
+  for (int i = threadIdx.x; i < D::kEdgeTable; i += 256)
+    tab[i] = mesh.table[D::kEdge + i];
+  __syncthreads();
+
+  static_assert(D::kVTable <= kVTableMax, "volume table slot too small");
+  const double *vt = g_vtable[P];
+  double(*xq)[4] = xs[quad][0];
+  double(*xo)[4] = xs[quad][1];
+  double(*ff)[8] = xf[quad];
+
+  // Volume integral, four points per quad and iteration.
+  if constexpr (P > 0) {
+#pragma unroll 1
+    for (int it = 0; it < NIT; ++it) {
+      const int q0 = (it * H + h) * 4;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double *b = vt + D::kVB + (q0 + j) * Np;
+        double x = 0.0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          x += c[m] * b[m];
+        xq[j][v] = x;
+      }
+      lane_exchange_fence();
+      {
+        // lane v evaluates the flux at point q0 + v
+        const double s0 = xq[v][0], s1 = xq[v][1], s2 = xq[v][2], s3 = xq[v][3];
+        const double inv = rcp(s0);
+        const double u = s1 * inv, w = s2 * inv;
+        const double p = g1 * (s3 - 0.5 * (s1 * u + s2 * w));
+        const double fx[4] = {s1, s1 * u + p, s2 * u, (s3 + p) * u};
+        const double fy[4] = {s2, s1 * w, s2 * w + p, (s3 + p) * w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ff[v][k] = gA * fx[k] + gB * fy[k];
+          ff[v][4 + k] = gC * fx[k] + gD * fy[k];
+        }
+      }
+      lane_exchange_fence();
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double fxi = ff[j][v], feta = ff[j][4 + v];
+        const double *wx = vt + D::kVWx + (q0 + j) * Np;
+        const double *wy = vt + D::kVWy + (q0 + j) * Np;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          acc[m] += wx[m] * fxi + wy[m] * feta;
+      }
+      lane_exchange_fence();
+    }
+  }
+
+  // Faces: quad h takes edges h, h + H, ...; lane v builds both traces of
+  // variable v at the edge points, lane q < NE evaluates HLLC at point q.
+#pragma unroll 1
+  for (int k = h; k < 3; k += H) {
+    const int meta = mesh.meta[k * E + e];
+    const int nb = mesh.nbr[k * E + e];
+    const double nx = mesh.fn[(3 * k) * E + e];
+    const double ny = mesh.fn[(3 * k + 1) * E + e];
+    const double scale = mesh.fn[(3 * k + 2) * E + e];
+    const bool is_left = meta & kMetaLeft;
+    const bool boundary = meta & kMetaBoundary;
+    const int kl = meta_kl(meta), kr = meta_kr(meta), fl = meta_flip(meta);
+    const double *bl = tab + (kl * NE * Np);
+    const double *br = tab + (D::kBr - D::kEdge) + ((kl * 3 + kr) * 2 + fl) * NE * Np;
+    const double *own = is_left ? bl : br;
+    const double *oth = is_left ? br : bl;
+    double nv[Np];
+    if (!boundary) {
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        nv[m] = a.uin[(v * Np + m) * E + nb];
+    }
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      double si = 0.0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        si += c[m] * own[q * Np + m];
+      double so = 0.0;
+      if (!boundary) {
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          so += nv[m] * oth[q * Np + m];
+      } else if (mesh.bdonor) {
+        const int slot = meta_slot(meta);
+        const int d = mesh.bdonor[slot * NE + q];
+        const double *db = mesh.bbasis + (slot * NE + q) * Np;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          so += a.uin[(v * Np + m) * E + d] * db[m];
+      } else {
+        so = si;  // transmissive: exterior = own trace
+      }
+      xq[q][v] = si;
+      xo[q][v] = so;
+    }
+    lane_exchange_fence();
+    if (v < NE) {
+      const C4 si = {xq[v][0], xq[v][1], xq[v][2], xq[v][3]};
+      const C4 so = {xo[v][0], xo[v][1], xo[v][2], xo[v][3]};
+      const C4 L = is_left ? si : so;
+      const C4 R = is_left ? so : si;
+      const C4 f = hllc(g, L, R, nx, ny);
+      const double wq = tab[(D::kWe - D::kEdge) + kl * NE + v] * scale;
+      const double sw = is_left ? -wq : wq;
+      ff[v][0] = f.r * sw;
+      ff[v][1] = f.mx * sw;
+      ff[v][2] = f.my * sw;
+      ff[v][3] = f.E * sw;
+    }
+    lane_exchange_fence();
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const double val = ff[q][v];
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        acc[m] += val * own[q * Np + m];
+    }
+    lane_exchange_fence();
+  }
+
+  if constexpr (H == 2) {
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      acc[m] += __shfl_xor(acc[m], 4, 64);
+  }
+
+  // Finalise variable v (quad 0 of the element stores).
+  const double invdet = mesh.geo[4 * E + e];
+  double y[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    const double r = acc[m] * invdet;
+    const int idx = (v * Np + m) * E + e;
+    if (a.residual_only) {
+      y[m] = r;
+    } else {
+      y[m] = a.cb * (c[m] + dt * r);
+      if (a.ca != 0.0)
+        y[m] += a.ca * a.un[idx];
+    }
+    if (valid && h == 0)
+      a.out[idx] = y[m];
+  }
+
+  // New cell averages to every lane of the quad.
+  const double o0 = dpp<quad_bcast<0>()>(y[0]);
+  const double o1 = dpp<quad_bcast<1>()>(y[0]);
+  const double o2 = dpp<quad_bcast<2>()>(y[0]);
+  const double o3 = dpp<quad_bcast<3>()>(y[0]);
+
+  if (a.sensor && !a.residual_only && v == 0 && h == 0 && valid) {
+    // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
+    double total = 0.0, high = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      total += y[m] * y[m];
+      if (m >= a.highest_begin)
+        high += y[m] * y[m];
+    }
+    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
+    if (a.flags)
+      a.flags[e] = troubled ? 1 : 0;
+    if (troubled)
+      a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+  }
+
+  if (a.want_lambda && !a.residual_only) {
+    // |u| + c of the new cell average (cfl_timestep.cpp:33-47).
+    double speed = 0.0;
+    if (v == 0 && h == 0 && valid) {
+      const double is2 = 1.0 / sqrt(2.0);
+      const double r = is2 * o0, mx = is2 * o1, my = is2 * o2, en = is2 * o3;
+      const double inv = 1.0 / r;
+      const double u = mx * inv, w = my * inv;
+      const double p = g1 * (en - 0.5 * (mx * u + my * w));
+      if (!(r > 0.0 && p > 0.0)) {
+        if (ctl)
+          ctl->s[a.parity ^ 1].bad = 1;
+      } else {
+        speed = hypot(u, w) + sqrt(g * p / r);
+      }
+    }
+    speed = wave_max(speed);
+    if ((threadIdx.x & 63) == 0)
+      red[threadIdx.x >> 6] = speed;
+    __syncthreads();
+    if (threadIdx.x == 0 && ctl) {
+      const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      atomicMax(&ctl->s[a.parity ^ 1].lambda,
+                static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
   }
   stage_epilogue(ctl, a.parity, a.stage, dt);

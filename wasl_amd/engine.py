@@ -55,6 +55,7 @@ class Numerics(C.Structure):
         ("pressure_floor", C.c_double),
         ("precision", C.c_int32),
         ("lanes_per_element", C.c_int32),
+        ("stage_kernel", C.c_int32),
     ]
 
 
@@ -259,7 +260,8 @@ class Engine:
                  wavespeed: float = 0.0, boundary=(PERIODIC, PERIODIC),
                  boundary_values=(0.0, 0.0), limiting: bool = False,
                  sensor_threshold: float = 1e-3, density_floor: float = 1e-10,
-                 pressure_floor: float = 1e-10, precision: int = FP64, lanes: int = 0):
+                 pressure_floor: float = 1e-10, precision: int = FP64, lanes: int = 0,
+                 stage_kernel: int = 0):
         lib = library()
         n = Numerics()
         lib.wasl_hip_numerics_default(C.byref(n))
@@ -275,6 +277,7 @@ class Engine:
         n.pressure_floor = pressure_floor
         n.precision = precision
         n.lanes_per_element = lanes
+        n.stage_kernel = stage_kernel
         self.numerics = n
         h = _P()
         _check(lib.wasl_hip_engine_create(mesh.h, C.byref(n), C.byref(h)))
@@ -284,6 +287,7 @@ class Engine:
         self.V = lib.wasl_hip_engine_size(self.h, 2)
         self.F = lib.wasl_hip_engine_size(self.h, 4)
         self.lanes = lib.wasl_hip_engine_size(self.h, 6)
+        self.stage_kernel = lib.wasl_hip_engine_size(self.h, 8)
         self.order = order
         self.dimension = mesh.dimension
 
