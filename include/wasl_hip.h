@@ -77,7 +77,7 @@ int wasl_hip_mesh_from_prepared(int dimension, int64_t node_count, const int64_t
                                 const int32_t *face_local_edges, const double *face_normals,
                                 wasl_hip_mesh **out);
 
-/* Counts: 0 elements, 1 faces, 2 boundary faces, 3 nodes. */
+/* Counts: 0 elements, 1 faces, 2 boundary faces, 3 nodes; 4 gives the dimension. */
 int64_t wasl_hip_mesh_count(const wasl_hip_mesh *mesh, int what);
 
 /* Prepared-mesh arrays of the canonical mesh (prepare_mesh, src/mesh/prepared/

@@ -1,8 +1,11 @@
-"""Builds wasl_amd/libwasl_hip.so in-tree for gfx950.
+"""Builds wasl_amd/libwasl_hip.so and wasl_amd/libwasl_hip_case.so in-tree.
 
 Host setup (setup.cpp) is compiled with g++ without FMA contraction so the
 reference-element tables hold the same doubles as the reference's own
 evaluations; device code (engine.hip) is compiled with hipcc for gfx950.
+The case layer (case_io.cpp, case_runner.cpp: the coarse calls, the runner
+and the HDF5 formats) is plain C++ over the engine's C ABI, linked against
+the HDF5 C library.
 """
 
 from __future__ import annotations
@@ -15,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwasl_hip.so")
+CASE_LIB = os.path.join(HERE, "libwasl_hip_case.so")
+HDF5 = os.environ.get("WASL_HIP_HDF5", "/opt/conda")
 ARCH = os.environ.get("WASL_HIP_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -42,7 +47,30 @@ def build(verbose: bool = True) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", engine_o, setup_o, "-o", tmp,
           "-lpthread"])
     os.replace(tmp, LIB)
+    build_case()
     return LIB
+
+
+def build_case() -> str:
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    for name in ("case_io", "case_runner"):
+        o = os.path.join(OUT, name + ".o")
+        _run(["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", f"-I{HDF5}/include",
+              "-c", os.path.join(CSRC, name + ".cpp"), "-o", o])
+        objs.append(o)
+    # A directory holding only the HDF5 library: putting HDF5's own lib
+    # directory on the run path would also put its older libstdc++ there.
+    h5dir = os.path.join(HERE, "_hdf5")
+    os.makedirs(h5dir, exist_ok=True)
+    link = os.path.join(h5dir, "libhdf5.so.103")
+    if not os.path.lexists(link):
+        os.symlink(os.path.join(HDF5, "lib", "libhdf5.so.103"), link)
+    tmp = CASE_LIB + ".tmp"
+    _run(["g++", "-shared", "-fPIC", *objs, "-o", tmp, f"-L{HERE}", "-lwasl_hip",
+          f"-L{h5dir}", "-l:libhdf5.so.103", "-Wl,-rpath,$ORIGIN:$ORIGIN/_hdf5"])
+    os.replace(tmp, CASE_LIB)
+    return CASE_LIB
 
 
 if __name__ == "__main__":

@@ -714,6 +714,8 @@ int64_t wasl_hip_mesh_count(const wasl_hip_mesh *mesh, int what) {
     std::sort(tags.begin(), tags.end());
     return static_cast<int64_t>(std::unique(tags.begin(), tags.end()) - tags.begin());
   }
+  case 4:
+    return m.dimension;
   default:
     return -1;
   }
