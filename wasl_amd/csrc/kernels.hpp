@@ -276,6 +276,17 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
   return true;
 }
 
+// Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b % 8),
+// each XCD with its own L2. This renumbering gives every XCD one contiguous
+// range of workgroups, so the face neighbours of an element, which the mesh
+// numbers close to it, are mostly read through the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int n = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  const int x = b % kXcds, i = b / kXcds, q = n / kXcds, r = n % kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // The last stage of a step advances time and step count for the next step.
 __device__ inline void stage_epilogue(RunCtl *ctl, int parity, int stage, double dt) {
   if (ctl && stage == 2 && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -323,23 +334,18 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
   __shared__ double tab[D::kTable];
   __shared__ double red[4];
 
-  RunCtl *ctl = a.ctl;
-  double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt))
-    return;
-  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
-    a.counters[(a.stage + 1) % 3] = 0;
-
   const int E = mesh.E;
   const int sub = threadIdx.x % LANES;
-  const int e_raw = blockIdx.x * PER_BLOCK + threadIdx.x / LANES;
+  const int e_raw = xcd_block() * PER_BLOCK + threadIdx.x / LANES;
   const bool valid = e_raw < E;
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
 
-  // Global requests first: own coefficients and, with one edge per lane, the
-  // edge record and the neighbour's coefficients, so their latency overlaps
-  // the table staging and the volume integral.
+  // Global requests first, ahead of the step-control read: own coefficients
+  // and, with one edge per lane, the edge record and the neighbour's
+  // coefficients, so their latency overlaps the prologue, the table staging
+  // and the volume integral. Every address is in bounds whether or not the
+  // stage runs.
   double c[4][Np];
   load_coefficients<Np>(a.uin, E, e, c);
   constexpr bool kPrefetch = LANES == 4 && P <= 2;
@@ -351,6 +357,13 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
     nb0 = mesh.nbr[k0 * E + e];
     load_coefficients<Np>(a.uin, E, nb0 >= 0 ? nb0 : e, nc);
   }
+
+  RunCtl *ctl = a.ctl;
+  double dt = a.dt;
+  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+    return;
+  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
 
   for (int i = threadIdx.x; i < D::kTable; i += 256)
     tab[i] = mesh.table[i];
@@ -593,7 +606,7 @@ __device__ __forceinline__ void lane_exchange_fence() {
 // the scalar cache; edge tables depend on each element's face configuration
 // and are staged in LDS.
 template <int P, int H>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
 void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   using D = Tri<P>;
   constexpr int Np = D::Np, NE = D::NE;
@@ -604,22 +617,16 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   __shared__ double xf[64][4][8];      // per quad: [point][F_xi 0..3, F_eta 0..3] or [point][F 0..3]
   __shared__ double red[4];
 
-  RunCtl *ctl = a.ctl;
-  double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt))
-    return;
-  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
-    a.counters[(a.stage + 1) % 3] = 0;
-
   const int E = mesh.E;
   const int v = threadIdx.x & 3;              // owned variable
   const int quad = threadIdx.x >> 2;          // quad slot in the block
   const int h = quad % H;                     // quad index within the element
-  const int e_raw = blockIdx.x * PER_BLOCK + quad / H;
+  const int e_raw = xcd_block() * PER_BLOCK + quad / H;
   const bool valid = e_raw < E;
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
 
+  // Loads ahead of the step-control read (all in bounds).
   double c[Np], acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -628,6 +635,13 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   }
   const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
                gD = mesh.geo[3 * E + e];
+
+  RunCtl *ctl = a.ctl;
+  double dt = a.dt;
+  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+    return;
+  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
 
   for (int i = threadIdx.x; i < D::kEdgeTable; i += 256)
     tab[i] = mesh.table[D::kEdge + i];
