@@ -96,6 +96,72 @@ def cpu_baseline(mesh, order, cfl, budget_s: float):
     }
 
 
+def serving_measurement(count: int, seed: int) -> dict:
+    """The serving mix run plain (FP64, one case at a time) and served
+    (mixed precision, continuous batching): aggregate DOF-updates/s of each,
+    and time to solution under Poisson arrivals at 80 % of the served
+    capacity measured closed-loop."""
+    import numpy as np
+
+    from wasl_amd import workload as WL
+
+    mix = WL.serving_mix(count, seed)
+    meshes = WL.MeshCache()
+    for r in mix:
+        meshes.get(r.mesh_key)
+    log(f"serving mix: {count} requests ({sum(r.kind != 'advection' for r in mix)} Euler)")
+    plain_total = WL.run_plain(mix, meshes)
+    updates = sum(r.dofs * r.plain_steps for r in mix)
+    plain_rate = updates / plain_total
+    log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s")
+
+    done, makespan = WL.run_served(mix, meshes)
+    served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
+    served_rate = served_updates / makespan
+    capacity = count / makespan
+    log(f"served closed-loop: {makespan:.2f} s, {served_rate:.4e} DOF-updates/s, "
+        f"x{served_rate / plain_rate:.2f} plain")
+
+    rng = np.random.default_rng(seed + 1)
+    rate = 0.8 * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
+    done_o, makespan_o = WL.run_served(mix, meshes, arrivals=arrivals)
+    tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
+    plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
+
+    worst = 0.0
+    for k, r in enumerate(mix):
+        if r.kind == "advection":
+            continue
+        Np = (r.order + 1) * (r.order + 2) // 2
+        E = r.size
+        a = done[k][3].reshape(4, Np, E)[0]
+        b = r.plain_state.reshape(4, Np, E)[0]
+        worst = max(worst, float(np.max(np.abs(a - b))))
+    out = {
+        "requests": count,
+        "mix": "70% 2D Euler (vortex | radial rarefaction, limited | smooth Fourier), "
+               "30% 1D advection; tris log-U[512,131072], cells log-U[256,65536], P U{1,2,3}, "
+               "200-2000 steps",
+        "plain_dof_updates_per_s": plain_rate,
+        "served_dof_updates_per_s": served_rate,
+        "served_vs_plain": served_rate / plain_rate,
+        "served_precision": "mixed: FP32 matrix-core contractions, FP64 state/RK/limiter",
+        "capacity_requests_per_s": capacity,
+        "arrival_rate_per_s": rate,
+        "tts_p50_s": WL.percentile(tts, 50),
+        "tts_p99_s": WL.percentile(tts, 99),
+        "plain_tts_p50_s": WL.percentile(plain_tts, 50),
+        "plain_tts_p99_s": WL.percentile(plain_tts, 99),
+        "open_loop_makespan_s": makespan_o,
+        "density_linf_served_vs_fp64": worst,
+    }
+    log(f"served open-loop at {rate:.2f} req/s: TTS p50 {out['tts_p50_s']:.3f} s "
+        f"p99 {out['tts_p99_s']:.3f} s (plain p50 {out['plain_tts_p50_s']:.3f} s "
+        f"p99 {out['plain_tts_p99_s']:.3f} s); density L-inf vs FP64 {worst:.2e}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +172,12 @@ def main(argv=None) -> int:
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--stage-kernel", type=int, default=0,
                     help="0 autotuned, 1 point split, 2 variable split")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "bf16"], default="fp64",
+                    help="fp64: the plain drop-in; mixed: FP32 matrix-core contractions; "
+                         "bf16: the BF16 ablation")
+    ap.add_argument("--serving-requests", type=int, default=64,
+                    help="requests of the serving mix (0: skip the serving measurement)")
+    ap.add_argument("--serving-seed", type=int, default=0x5EED)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -139,8 +211,10 @@ def main(argv=None) -> int:
     p = args.order
     mesh_in = MG.jittered_triangles(args.cells, (0.0, 0.0), (10.0, 10.0), 0.15)
     mesh = wasl_amd.Mesh.from_input(mesh_in)
+    precision = {"fp64": wasl_amd.FP64, "mixed": wasl_amd.MIXED, "bf16": wasl_amd.BF16}[
+        args.precision]
     eng = wasl_amd.Engine(mesh, wasl_amd.EULER, p, lanes=args.lanes,
-                          stage_kernel=args.stage_kernel)
+                          stage_kernel=args.stage_kernel, precision=precision)
     eng.project(wasl_amd.IC_VORTEX, (5.0, 5.0, 5.0))
     E, Np, V = eng.E, eng.Np, eng.V
     log(f"engine: {E} triangles, P{p}, stage kernel {eng.stage_kernel}, "
@@ -190,7 +264,8 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64" if args.precision == "mixed"
+                 else "bf16+f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"2D Euler isentropic vortex, P{p} modal Dubiner, "
@@ -198,7 +273,8 @@ def main(argv=None) -> int:
                         f"HLLC, SSP-RK3, CFL {cfl}, device dt, plain FP64",
             "elements": E, "modes": Np, "variables": V,
             "dofs": E * Np * V, "lanes_per_element": eng.lanes,
-            "stage_kernel": ["point split", "variable split"][eng.stage_kernel - 1],
+            "stage_kernel": ("matrix-core mixed" if args.precision != "fp64" else
+                             ["point split", "variable split"][eng.stage_kernel - 1]),
             "stage_rate": 3 * args.steps / wall, "state_finite": finite,
         },
         "roofline": {
@@ -208,7 +284,8 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1],
+            "kernel": ("whip::euler_stage_mx" if args.precision != "fp64" else
+                       ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1]),
             "stage_kernel_us": stage_ms * 1e3,
             "algorithmic_bytes_per_launch": bytes_stage,
             "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,
@@ -216,6 +293,9 @@ def main(argv=None) -> int:
         },
         "cpu_baseline": None,
     }
+    if args.serving_requests > 0:
+        del eng
+        result["serving"] = serving_measurement(args.serving_requests, args.serving_seed)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
         result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)

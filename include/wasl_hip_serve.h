@@ -1,0 +1,89 @@
+/*
+ * wasl_hip serving: many resident cases advanced together on one GPU.
+ *
+ * No counterpart in the reference, which runs one case per process
+ * (src/case/serial_case_runner.cpp:117-149). A server takes requests (a mesh,
+ * numerics as for an engine, an initial condition and an end time), prepares
+ * each on a host worker thread, and admits prepared cases while device slots
+ * and device memory allow (continuous batching). Each scheduling round
+ * (wasl_hip_server_pump) advances every resident case by up to steps_per_batch
+ * SSP-RK3 steps with one batched kernel launch per stage and class (equation,
+ * polynomial order), each case taking its own CFL dt on the device and stopping
+ * at its own end time; finished cases are retired and their slots reused.
+ * A captured HIP graph serves every round with the same resident set.
+ *
+ * A case computes exactly what it computes alone: tiles of the batched
+ * kernels are the blocks of the single-case kernel, so results do not depend
+ * on the other resident cases.
+ *
+ * 2D Euler cases run the stage in the server's precision (WASL_HIP_MIXED by
+ * default, or WASL_HIP_BF16); the limiter, the CFL speed and the state stay
+ * FP64. 1D advection cases run in FP64.
+ *
+ * Threading: one thread drives a server; the server owns its worker threads.
+ */
+#ifndef WASL_HIP_SERVE_H
+#define WASL_HIP_SERVE_H
+
+#include <stdint.h>
+
+#include "wasl_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wasl_hip_server wasl_hip_server;
+
+typedef struct {
+  int32_t precision;        /* stage arithmetic of 2D Euler cases: WASL_HIP_MIXED or WASL_HIP_BF16 */
+  int32_t max_resident;     /* device slots (default 256) */
+  uint64_t memory_budget;   /* device bytes for prepared and resident cases; 0: 80% of free */
+  int32_t steps_per_batch;  /* steps per scheduling round, even (default 16) */
+  int32_t use_graphs;       /* capture one graph per resident set (default 1) */
+  int32_t setup_threads;    /* host threads preparing cases (default 4, at most 16) */
+} wasl_hip_server_config;
+
+void wasl_hip_server_config_default(wasl_hip_server_config *config);
+int wasl_hip_server_create(const wasl_hip_server_config *config, wasl_hip_server **out);
+/* Waits for the worker threads; pending and unclaimed requests are dropped. */
+void wasl_hip_server_destroy(wasl_hip_server *server);
+
+/* Queues one case. The mesh is copied (it may be destroyed afterwards); the
+ * numerics are those of wasl_hip_engine_create (the server's precision applies
+ * to 2D Euler); ic/params as wasl_hip_project. The case runs to end_time, or,
+ * when end_steps > 0, to end_steps times the CFL step of its projected initial
+ * state; max_steps bounds the step count. */
+int wasl_hip_server_submit(wasl_hip_server *server, const wasl_hip_mesh *mesh,
+                           const wasl_hip_numerics *numerics, int ic, const double *params,
+                           int64_t param_count, double cfl, double end_time, uint64_t end_steps,
+                           uint64_t max_steps, uint64_t *id);
+
+/* One scheduling round: start preparing queued cases that fit the budget,
+ * admit prepared ones, advance the resident set by up to steps_per_batch steps
+ * and retire finished cases. Returns up to capacity finished request ids. When
+ * nothing is resident it waits (up to 50 ms) for a preparation to finish. */
+int wasl_hip_server_pump(wasl_hip_server *server, uint64_t *completed, int64_t capacity,
+                         int64_t *count);
+
+/* Outcome of a finished request: the final state (the engine's layout; pass
+ * NULL to skip), time and step, and the case's status (0, or the engine
+ * status of its failure, with the message from wasl_hip_last_error()).
+ * Releases the request. */
+int wasl_hip_server_result(wasl_hip_server *server, uint64_t id, double *state,
+                           int64_t state_capacity, double *time, uint64_t *step,
+                           int *case_status);
+
+/* Size of a request's state in doubles (-1 if unknown). */
+int64_t wasl_hip_server_state_size(const wasl_hip_server *server, uint64_t id);
+
+/* Counters: 0 queued, 1 preparing or prepared, 2 resident, 3 finished and
+ * unclaimed, 4 device bytes reserved, 5 graphs captured, 6 rounds run,
+ * 7 resident-set changes. */
+int64_t wasl_hip_server_count(const wasl_hip_server *server, int what);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WASL_HIP_SERVE_H */

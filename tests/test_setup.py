@@ -122,9 +122,13 @@ def test_meshgen_deterministic_and_bounded():
 
 
 def _declared_symbols():
-    text = open(os.path.join(ROOT, "include", "wasl_hip.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(wasl_hip_\w+)\s*\(", text)))
+    """Entry points of the headers libwasl_hip.so implements (engine, serving)."""
+    names = set()
+    for header in ("wasl_hip.h", "wasl_hip_serve.h"):
+        text = open(os.path.join(ROOT, "include", header)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(wasl_hip_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_cabi_exports_every_declared_symbol(wasl):

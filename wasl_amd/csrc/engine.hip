@@ -128,7 +128,7 @@ struct wasl_hip_engine {
   // state and stage buffers
   Buffer u, w1, w2;
   // geometry and tables
-  Buffer geo, nbr, meta, fn, bdonor, bbasis, table, vtable, inv_jac;
+  Buffer geo, nbr, meta, fn, bdonor, bbasis, table, vtable, inv_jac, mxtable;
   // limiter
   Buffer lnbr, T, nrm, check, flags, list, counters, side, err;
   // control and reductions
@@ -270,6 +270,85 @@ std::vector<double> euler_vtables(int p) {
   return t;
 }
 
+uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, sizeof u);
+  return static_cast<uint16_t>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Per-lane operand tables of the mixed-precision stage kernel (whip::Mx<P>):
+// every entry is the value the lane of that slot feeds to the matrix core, so
+// the kernel reads tab[offset + lane].
+template <int P> std::vector<float> mixed_tables() {
+  using X = whip::Mx<P>;
+  constexpr int Np = X::Np, NQ = X::NQ, NE = X::NE, MT = X::MT, KS = X::KS;
+  std::vector<float> t(X::kTable, 0.0f);
+  const whip::TriRule vol = whip::triangle_rule(P + 2);
+  std::vector<double> B(NQ * Np), Wx(NQ * Np), Wy(NQ * Np), b(Np), gx(Np), ge(Np);
+  for (int q = 0; q < NQ; ++q) {
+    whip::dubiner(vol.xi[q], vol.eta[q], P, b.data());
+    whip::dubiner_gradient(vol.xi[q], vol.eta[q], P, gx.data(), ge.data());
+    for (int m = 0; m < Np; ++m) {
+      B[q * Np + m] = b[m];
+      Wx[q * Np + m] = vol.w[q] * gx[m];
+      Wy[q * Np + m] = vol.w[q] * ge[m];
+    }
+  }
+  const auto vb = [&](int q, int m) { return q < NQ && m < Np ? B[q * Np + m] : 0.0; };
+  const auto wd = [&](int d, int q, int m) {
+    return q < NQ && m < Np ? (d == 0 ? Wx : Wy)[q * Np + m] : 0.0;
+  };
+  const auto edges = whip::edge_rules(P);
+  std::vector<double> Be(3 * NE * Np);
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < NE; ++s)
+      whip::dubiner(edges[k].xi[s], edges[k].eta[s], P, Be.data() + (k * NE + s) * Np);
+  const auto eb = [&](int k, int s, int m) {
+    return k < 3 && s < NE && m < Np ? Be[(k * NE + s) * Np + m] : 0.0;
+  };
+  for (int l = 0; l < 64; ++l) {
+    const int i = l & 15, kk = l >> 4;
+    for (int tt = 0; tt < MT; ++tt) {
+      for (int s = 0; s < KS; ++s)
+        t[X::kAvol + (tt * KS + s) * 64 + l] = static_cast<float>(vb(16 * tt + i, 4 * s + kk));
+      for (int r = 0; r < 4; ++r)
+        for (int d = 0; d < 2; ++d)
+          t[X::kAgrad + ((tt * 4 + r) * 2 + d) * 64 + l] =
+              static_cast<float>(wd(d, 16 * tt + 4 * kk + r, i));
+      // bf16: A[i][4 kk + c] with four consecutive K entries per lane
+      uint16_t hv[4], hx[4], he[4];
+      for (int c = 0; c < 4; ++c) {
+        hv[c] = bf16_bits(static_cast<float>(vb(16 * tt + i, 4 * kk + c)));
+        hx[c] = bf16_bits(static_cast<float>(wd(0, 16 * tt + 4 * kk + c, i)));
+        he[c] = bf16_bits(static_cast<float>(wd(1, 16 * tt + 4 * kk + c, i)));
+      }
+      std::memcpy(&t[X::kBvol16 + (tt * 64 + l) * 2], hv, sizeof hv);
+      std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 0) * 64 + l) * 2], hx, sizeof hx);
+      std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 1) * 64 + l) * 2], he, sizeof he);
+    }
+    for (int s = 0; s < KS; ++s)
+      t[X::kAface + s * 64 + l] = static_cast<float>(eb(i >> 2, i & 3, 4 * s + kk));
+    for (int s = 0; s < NE; ++s)
+      t[X::kAlift + s * 64 + l] = static_cast<float>(eb(kk, s, i));
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int s = 0; s < NE; ++s) {
+      for (int m = 0; m < Np; ++m)
+        t[X::kBe + (k * NE + s) * Np + m] = static_cast<float>(Be[(k * NE + s) * Np + m]);
+      t[X::kWe + k * NE + s] = static_cast<float>(edges[k].w[s]);
+    }
+  return t;
+}
+
+std::vector<float> mixed_tables(int p) {
+  switch (p) {
+  case 0: return mixed_tables<0>();
+  case 1: return mixed_tables<1>();
+  case 2: return mixed_tables<2>();
+  default: return mixed_tables<3>();
+  }
+}
+
 std::vector<double> line_tables(int p) {
   const int Np = p + 1, NQ = p > 0 ? p : 1;
   std::vector<double> t(2 * NQ * Np + 2 * Np, 0.0);
@@ -293,6 +372,16 @@ std::vector<double> line_tables(int p) {
 template <int P>
 void launch_euler_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
   const whip::EulerMesh m = g.euler_mesh();
+  if (g.num.precision != WASL_HIP_FP64) {
+    // 64 elements per 256-thread block (16 per wave)
+    const dim3 grid((g.E + 63) / 64);
+    const float *t = g.mxtable.as<float>();
+    if (g.num.precision == WASL_HIP_BF16)
+      hipLaunchKernelGGL((whip::euler_stage_mx<P, true>), grid, dim3(256), 0, g.stream, m, a, t);
+    else
+      hipLaunchKernelGGL((whip::euler_stage_mx<P, false>), grid, dim3(256), 0, g.stream, m, a, t);
+    return;
+  }
   const int per = g.stage_kernel == 2 ? 256 / g.lanes : 256 / g.lanes;
   const dim3 grid((g.E + per - 1) / per);
   if (g.stage_kernel == 2) {
@@ -482,8 +571,9 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   g.p = num.polynomial_order;
   if (g.p < 0 || g.p > 3)
     throw std::runtime_error("engine: polynomial order must be 0..3");
-  if (num.precision != WASL_HIP_FP64)
-    throw std::runtime_error("engine: only the FP64 precision is available in this build");
+  if (num.precision != WASL_HIP_FP64 && num.precision != WASL_HIP_MIXED &&
+      num.precision != WASL_HIP_BF16)
+    throw std::runtime_error("engine: precision must be FP64, MIXED or BF16");
   g.mesh = mesh;
   HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   if (const char *env = std::getenv("WASL_HIP_GRAPHS"))
@@ -534,6 +624,7 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     std::vector<double> fn(9 * E, 0.0);
     std::vector<int> bdon;
     std::vector<double> bbas;
+    const auto edges = whip::edge_rules(g.p);
     int slot = 0;
     for (size_t f = 0; f < mesh.faces.size(); ++f) {
       const whip::TriFace &x = mesh.faces[f];
@@ -563,8 +654,20 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
         }
         ++slot;
       } else {
-        put(x.left, x.left_edge, x.right, m | whip::kMetaLeft);
-        put(x.right, x.right_edge, x.left, m);
+        // Whether the right side lists the face's Gauss points in the reverse
+        // order of the left side (each in its own edge rule), by position.
+        int rev = 0;
+        if (NE > 1) {
+          const whip::Triangle &r = mesh.tris[x.right];
+          double lx, ly, r0x, r0y, r1x, r1y;
+          l.to_physical(edges[x.left_edge].xi[0], edges[x.left_edge].eta[0], lx, ly);
+          r.to_physical(edges[x.right_edge].xi[0], edges[x.right_edge].eta[0], r0x, r0y);
+          r.to_physical(edges[x.right_edge].xi[NE - 1], edges[x.right_edge].eta[NE - 1], r1x, r1y);
+          rev = std::hypot(lx - r0x, ly - r0y) > std::hypot(lx - r1x, ly - r1y) ? 1 : 0;
+        }
+        const int rb = rev ? whip::kMetaReverse : 0;
+        put(x.left, x.left_edge, x.right, m | whip::kMetaLeft | rb);
+        put(x.right, x.right_edge, x.left, m | rb);
       }
     }
     if (std::find(nbr.begin(), nbr.end(), -2) != nbr.end())
@@ -578,6 +681,8 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     upload(g.bdonor, bdon, tot, g.stream);
     upload(g.bbasis, bbas, tot, g.stream);
     upload(g.table, euler_tables(g.p), tot, g.stream);
+    if (num.precision != WASL_HIP_FP64)
+      upload(g.mxtable, mixed_tables(g.p), tot, g.stream);
     {
       // Reference-element tables of order p into the constant slot of that
       // order; every engine of the same order writes the same values.
@@ -1233,3 +1338,6 @@ int wasl_hip_set_timing(wasl_hip_engine *g, int on) {
 }
 
 }  // extern "C"
+
+// The serving layer shares this translation unit (engine internals, kernels).
+#include "server.inl"

@@ -225,7 +225,8 @@ __device__ inline C4 hllc(double gamma, const C4 &L, const C4 &R, double nx, dou
 // gathered at the end of the previous step, and publishes it for stages 1-2.
 // Every block evaluates the same decision from the same data, so the block
 // exits are uniform. Returns false when the stage must not run.
-__device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double &dt) {
+__device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double &dt,
+                                      bool leader) {
   if (!ctl)
     return true;
   if (ctl->err)
@@ -240,7 +241,7 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
   const double time = cur.time;
   const bool active = time < ctl->end_time && cur.step < ctl->max_steps;
   if (!active) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (leader && threadIdx.x == 0) {
       cur.active = 0;
       StepSlot &nxt = ctl->s[parity ^ 1];
       nxt.time = time;
@@ -265,7 +266,7 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
       raise_error(ctl, nullptr, kErrTimestep);
     return false;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (leader && threadIdx.x == 0) {
     cur.dt = dt;
     cur.active = 1;
     ctl->last_dt = dt;
@@ -288,8 +289,9 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 // The last stage of a step advances time and step count for the next step.
-__device__ inline void stage_epilogue(RunCtl *ctl, int parity, int stage, double dt) {
-  if (ctl && stage == 2 && blockIdx.x == 0 && threadIdx.x == 0) {
+__device__ inline void stage_epilogue(RunCtl *ctl, int parity, int stage, double dt,
+                                      bool leader) {
+  if (ctl && stage == 2 && leader && threadIdx.x == 0) {
     StepSlot &cur = ctl->s[parity];
     StepSlot &nxt = ctl->s[parity ^ 1];
     nxt.time = cur.time + dt;
@@ -360,7 +362,7 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+  if (!stage_prologue(ctl, a.parity, a.stage, dt, blockIdx.x == 0))
     return;
   if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
@@ -576,7 +578,7 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
                   static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
   }
-  stage_epilogue(ctl, a.parity, a.stage, dt);
+  stage_epilogue(ctl, a.parity, a.stage, dt, blockIdx.x == 0);
 }
 
 // ------------------------------------------- 2D Euler, variable split ----
@@ -638,7 +640,7 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt))
+  if (!stage_prologue(ctl, a.parity, a.stage, dt, blockIdx.x == 0))
     return;
   if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
@@ -839,7 +841,470 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
                 static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
   }
-  stage_epilogue(ctl, a.parity, a.stage, dt);
+  stage_epilogue(ctl, a.parity, a.stage, dt, blockIdx.x == 0);
+}
+
+// -------------------------------------- 2D Euler, mixed precision (MFMA) ----
+//
+// The same fused stage with the contractions in FP32 on the matrix cores.
+// A wave serves 16 consecutive elements, one per matrix column: lane l owns
+// element j = l & 15 of the wave and row group g = l >> 4, and the four
+// conserved variables are four products sharing one A operand. Operand
+// layouts of v_mfma_f32_16x16x4f32 on gfx950:
+//   A (16 x 4):  lane l holds A[l & 15][l >> 4]
+//   B (4 x 16):  lane l holds B[l >> 4][l & 15]
+//   D (16 x 16): lane l holds D[4 (l >> 4) + r][l & 15], r = 0..3
+// and of v_mfma_f32_16x16x16bf16_1k: A[l & 15][4 (l >> 4) + i], B[4 (l >> 4) + i][l & 15]
+// for i = 0..3, D as above.
+//
+//   states at points  S_v[q][j]  = sum_m B_vol[q][m] c_v[m][j]
+//   volume term       R_v[m][j] += sum_q W_xi[q][m] F_xi,v[q][j] + W_eta[q][m] F_eta,v[q][j]
+//   own edge traces   T_v[4k+s][j] = sum_m B_edge[k][s][m] c_v[m][j]
+//   lifting           R_v[m][j] += sum_(k,s) B_edge[k][s][m] Fw_v[k][s][j]
+// A point tile of S comes out of the first product in exactly the layout the
+// second product takes as its B operand (K index = point 16t + 4g + r), and a
+// lane holds all four variables of its points, so the pointwise flux is
+// computed once per point with no exchange. Faces are evaluated in each
+// element's own edge frame: Gauss points are symmetric, so the two sides of a
+// face list the same physical points in the same or reversed order (meta bit
+// 7). Row group k serves edge k: neighbour traces, HLLC and the edge weights
+// run on the vector ALU in FP32, once per face point and side. The state stays
+// FP64 in HBM; the RK combination, sensor, CFL speed and limiter are FP64.
+
+constexpr int kMetaReverse = 128;   // the two sides order the face points oppositely
+
+template <int P> struct Mx {
+  static constexpr int Np = Tri<P>::Np, NQ = Tri<P>::NQ, NE = Tri<P>::NE;
+  static constexpr int MT = (NQ + 15) / 16;   // point tiles of 16
+  static constexpr int KS = (Np + 3) / 4;     // K steps of 4 modes
+  // per-lane operand tables, floats, each [..][64]
+  static constexpr int kAvol = 0;                         // [MT][KS]
+  static constexpr int kAgrad = kAvol + MT * KS * 64;     // [MT][4 r][2 d]
+  static constexpr int kAface = kAgrad + MT * 8 * 64;     // [KS]
+  static constexpr int kAlift = kAface + KS * 64;         // [NE]
+  static constexpr int kBe = kAlift + NE * 64;            // [3][NE][Np] own-edge basis
+  static constexpr int kWe = kBe + 3 * NE * Np;           // [3][NE] edge weights
+  // bf16 operands (pairs of bf16 per float slot), each lane 4 bf16 = 2 slots
+  static constexpr int kBvol16 = (kWe + 3 * NE + 3) / 4 * 4;   // [MT][64][2], 16-byte aligned
+  static constexpr int kBgrad16 = kBvol16 + MT * 128;     // [MT][2 d][64][2]
+  static constexpr int kTable = kBgrad16 + MT * 256;
+  static_assert(NE <= 4 && Np <= 16, "one edge per row group, modes in one tile");
+};
+
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+typedef short mx_s4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mx_f4 mfma4(float a, float b, mx_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Round to nearest even into bfloat16 (finite inputs).
+__device__ __forceinline__ short to_bf16(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return static_cast<short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <int CTRL> __device__ __forceinline__ float dppf(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+struct F4 {
+  float r, mx, my, E;
+};
+
+__device__ __forceinline__ float rcpf(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+
+struct SideF {
+  float inv_r, u, v, p, un, c;
+};
+
+__device__ __forceinline__ SideF side_of_f(float gamma, const F4 &s, float nx, float ny) {
+  SideF o;
+  o.inv_r = rcpf(s.r);
+  o.u = s.mx * o.inv_r;
+  o.v = s.my * o.inv_r;
+  o.p = (gamma - 1.0f) * (s.E - 0.5f * (s.mx * o.u + s.my * o.v));
+  o.un = o.u * nx + o.v * ny;
+  // FP32 rounding can push a positive pressure just below zero near vacuum
+  o.c = sqrtf(fmaxf(gamma * o.p * o.inv_r, 0.0f));
+  return o;
+}
+
+__device__ __forceinline__ F4 normal_flux_f(const F4 &s, const SideF &w, float nx, float ny) {
+  return {s.r * w.un, s.mx * w.un + w.p * nx, s.my * w.un + w.p * ny, (s.E + w.p) * w.un};
+}
+
+// HLLC (flux/hllc.hpp:40-74) in FP32.
+__device__ inline F4 hllc_f(float gamma, const F4 &L, const F4 &R, float nx, float ny) {
+  const SideF wl = side_of_f(gamma, L, nx, ny), wr = side_of_f(gamma, R, nx, ny);
+  const float sl = fminf(wl.un - wl.c, wr.un - wr.c);
+  const float sr = fmaxf(wl.un + wl.c, wr.un + wr.c);
+  if (sl >= 0.0f)
+    return normal_flux_f(L, wl, nx, ny);
+  if (sr <= 0.0f)
+    return normal_flux_f(R, wr, nx, ny);
+  const float sm = (wr.p - wl.p + L.r * wl.un * (sl - wl.un) - R.r * wr.un * (sr - wr.un)) *
+                   rcpf(L.r * (sl - wl.un) - R.r * (sr - wr.un));
+  const bool left = sm >= 0.0f;
+  const F4 &S = left ? L : R;
+  const SideF &w = left ? wl : wr;
+  const float s = left ? sl : sr;
+  const F4 F = normal_flux_f(S, w, nx, ny);
+  const float k = s - w.un;
+  const float rs = S.r * k * rcpf(s - sm);
+  const float shift = sm - w.un;
+  const float es = S.E * w.inv_r + shift * (sm + w.p * w.inv_r * rcpf(k));
+  return {F.r + s * (rs - S.r), F.mx + s * (rs * (w.u + shift * nx) - S.mx),
+          F.my + s * (rs * (w.v + shift * ny) - S.my), F.E + s * (rs * es - S.E)};
+}
+
+__device__ __forceinline__ mx_s4 bf16x4(const mx_f4 &x) {
+  return {to_bf16(x[0]), to_bf16(x[1]), to_bf16(x[2]), to_bf16(x[3])};
+}
+
+// Floats of the operand table a block stages in LDS.
+template <int P, bool BF16> constexpr int mx_staged() {
+  return BF16 ? Mx<P>::kTable : Mx<P>::kBvol16;
+}
+
+// One block (64 elements starting at block * 64) of the mixed stage; `leader`
+// marks the block that advances the case's step control.
+template <int P, bool BF16>
+__device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a,
+                                         const float *mxtab, int block, bool leader, float *tab,
+                                         double *red) {
+  using X = Mx<P>;
+  constexpr int Np = X::Np, NQ = X::NQ, NE = X::NE, MT = X::MT, KS = X::KS;
+  constexpr int kStaged = mx_staged<P, BF16>();
+
+  const int E = mesh.E;
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 15, grp = lane >> 4;
+  const int e_raw = block * 64 + (threadIdx.x >> 6) * 16 + j;
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+  const float g = static_cast<float>(mesh.gamma), g1 = g - 1.0f;
+
+  // Own coefficients as B operands (element j, modes grp + 4s; bf16: 4 grp + i),
+  // geometry, the edge record of edge `grp` and the neighbour's coefficients,
+  // all requested ahead of the step-control read (every address in bounds).
+  float cb[4][KS];
+  mx_s4 cb16[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int m = 4 * s + grp;
+      cb[v][s] = m < Np ? static_cast<float>(a.uin[(v * Np + m) * E + e]) : 0.0f;
+    }
+    if constexpr (BF16) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = 4 * grp + i;
+        cb16[v][i] = m < Np ? to_bf16(static_cast<float>(a.uin[(v * Np + m) * E + e])) : short(0);
+      }
+    }
+  }
+  const float gA = static_cast<float>(mesh.geo[e]), gB = static_cast<float>(mesh.geo[E + e]),
+              gC = static_cast<float>(mesh.geo[2 * E + e]),
+              gD = static_cast<float>(mesh.geo[3 * E + e]);
+  const int k = grp < 3 ? grp : 0;   // row group 3 idles on edge 0
+  const int meta = mesh.meta[k * E + e];
+  const int nbr = mesh.nbr[k * E + e];
+  const float nx = static_cast<float>(mesh.fn[(3 * k) * E + e]);
+  const float ny = static_cast<float>(mesh.fn[(3 * k + 1) * E + e]);
+  const float scale = static_cast<float>(mesh.fn[(3 * k + 2) * E + e]);
+  const bool boundary = meta & kMetaBoundary;
+  const int src = boundary ? e : nbr;
+
+  RunCtl *ctl = a.ctl;
+  double dt = a.dt;
+  if (!stage_prologue(ctl, a.parity, a.stage, dt, leader))
+    return;
+  if (a.sensor && leader && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
+
+  for (int i = threadIdx.x; i < kStaged; i += 256)
+    tab[i] = mxtab[i];
+  __syncthreads();
+
+  mx_f4 R[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    R[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // ---- volume: states at points (one product per variable, shared A),
+  // pointwise fluxes once per point, flux contraction
+  if constexpr (P > 0) {
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      mx_f4 S[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        S[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
+      if constexpr (BF16) {
+        const mx_s4 av = *reinterpret_cast<const mx_s4 *>(tab + X::kBvol16 + (t * 64 + lane) * 2);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          S[v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, cb16[v], S[v], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const float av = tab[X::kAvol + (t * KS + s) * 64 + lane];
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            S[v] = mfma4(av, cb[v][s], S[v]);
+        }
+      }
+      mx_f4 Fxi[4], Feta[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool real = 16 * t + 4 * grp + r < NQ;   // padded points: zero fluxes
+        const float rho = S[0][r], mx = S[1][r], my = S[2][r], en = S[3][r];
+        const float inv = rcpf(real ? rho : 1.0f);
+        const float u = mx * inv, w = my * inv;
+        const float p = g1 * (en - 0.5f * (mx * u + my * w));
+        const float fx[4] = {mx, mx * u + p, my * u, (en + p) * u};
+        const float fy[4] = {my, mx * w, my * w + p, (en + p) * w};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          Fxi[v][r] = real ? gA * fx[v] + gB * fy[v] : 0.0f;
+          Feta[v][r] = real ? gC * fx[v] + gD * fy[v] : 0.0f;
+        }
+      }
+      if constexpr (BF16) {
+        const mx_s4 ax =
+            *reinterpret_cast<const mx_s4 *>(tab + X::kBgrad16 + ((t * 2 + 0) * 64 + lane) * 2);
+        const mx_s4 ae =
+            *reinterpret_cast<const mx_s4 *>(tab + X::kBgrad16 + ((t * 2 + 1) * 64 + lane) * 2);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          R[v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ax, bf16x4(Fxi[v]), R[v], 0, 0, 0);
+          R[v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ae, bf16x4(Feta[v]), R[v], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float ax = tab[X::kAgrad + ((t * 4 + r) * 2 + 0) * 64 + lane];
+          const float ae = tab[X::kAgrad + ((t * 4 + r) * 2 + 1) * 64 + lane];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            R[v] = mfma4(ax, Fxi[v][r], R[v]);
+            R[v] = mfma4(ae, Feta[v][r], R[v]);
+          }
+        }
+      }
+    }
+  }
+
+  // ---- faces: own traces (rows 4k + s) on the matrix cores; neighbour
+  // traces, HLLC and weights on the vector ALU; lifting on the matrix cores.
+  float cn[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      cn[v][m] = static_cast<float>(a.uin[(v * Np + m) * E + src]);
+  mx_f4 T[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    T[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const float af = tab[X::kAface + s * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      T[v] = mfma4(af, cb[v][s], T[v]);
+  }
+
+  const bool is_left = meta & kMetaLeft;
+  const bool rev = meta & kMetaReverse;
+  const int kl = meta_kl(meta);
+  const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    F4 own = {T[0][s], T[1][s], T[2][s], T[3][s]};
+    F4 oth = own;   // transmissive: exterior = own trace
+    if (!boundary) {
+      const float *b = tab + X::kBe + (kn * NE + (rev ? NE - 1 - s : s)) * Np;
+      float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int m = 0; m < Np; ++m) {
+        const float bm = b[m];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          x[v] += cn[v][m] * bm;
+      }
+      oth = {x[0], x[1], x[2], x[3]};
+    } else if (mesh.bdonor) {
+      const int slot = meta_slot(meta);
+      const int d = mesh.bdonor[slot * NE + s];
+      const double *db = mesh.bbasis + (slot * NE + s) * Np;
+      float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int m = 0; m < Np; ++m) {
+        const float bm = static_cast<float>(db[m]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          x[v] += static_cast<float>(a.uin[(v * Np + m) * E + d]) * bm;
+      }
+      oth = {x[0], x[1], x[2], x[3]};
+    }
+    const F4 f = hllc_f(g, is_left ? own : oth, is_left ? oth : own, nx, ny);
+    const float wq = tab[X::kWe + kl * NE + s] * scale;
+    const float sw = is_left ? -wq : wq;
+    // Row group 3 has no edge: its (meaningless) flux is selected away, not
+    // multiplied by zero, since a NaN would survive the product and spread
+    // through the matrix core to every mode of the element.
+    const bool edge = grp < 3;
+    const float fw[4] = {edge ? f.r * sw : 0.0f, edge ? f.mx * sw : 0.0f, edge ? f.my * sw : 0.0f,
+                         edge ? f.E * sw : 0.0f};
+    const float al = tab[X::kAlift + s * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      R[v] = mfma4(al, fw[v], R[v]);
+  }
+
+  // ---- finalise: lane holds R[v][m = 4 grp + r] of element j, FP64 from here
+  const double invdet = mesh.geo[4 * E + e];
+  double total = 0.0, high = 0.0, avg[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 4 * grp + r;
+      if (m < Np) {
+        const double res = static_cast<double>(R[v][r]) * invdet;
+        const int idx = (v * Np + m) * E + e;
+        double y;
+        if (a.residual_only) {
+          y = res;
+        } else {
+          y = a.cb * (a.uin[idx] + dt * res);
+          if (a.ca != 0.0)
+            y += a.ca * a.un[idx];
+        }
+        if (valid)
+          a.out[idx] = y;
+        if (v == 0) {
+          total += y * y;
+          if (m >= a.highest_begin)
+            high += y * y;
+        }
+        if (m == 0)
+          avg[v] = y;
+      }
+    }
+  }
+
+  if (a.sensor && !a.residual_only) {
+    // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24): the
+    // element's density modes sit in its four row groups.
+    total += __shfl_xor(total, 16, 64);
+    high += __shfl_xor(high, 16, 64);
+    total += __shfl_xor(total, 32, 64);
+    high += __shfl_xor(high, 32, 64);
+    if (grp == 0 && valid) {
+      const bool troubled = !(total == 0.0 || high / total <= a.threshold);
+      if (a.flags)
+        a.flags[e] = troubled ? 1 : 0;
+      if (troubled)
+        a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+    }
+  }
+
+  if (a.want_lambda && !a.residual_only) {
+    // |u| + c of the new cell average (cfl_timestep.cpp:33-47), FP64.
+    double speed = 0.0;
+    if (grp == 0 && valid) {
+      const double is2 = 1.0 / sqrt(2.0);
+      const double r = is2 * avg[0], mx = is2 * avg[1], my = is2 * avg[2], en = is2 * avg[3];
+      const double inv = 1.0 / r;
+      const double u = mx * inv, w = my * inv;
+      const double p = (mesh.gamma - 1.0) * (en - 0.5 * (mx * u + my * w));
+      if (!(r > 0.0 && p > 0.0)) {
+        if (ctl)
+          ctl->s[a.parity ^ 1].bad = 1;
+      } else {
+        speed = hypot(u, w) + sqrt(mesh.gamma * p / r);
+      }
+    }
+    speed = wave_max(speed);
+    if ((threadIdx.x & 63) == 0)
+      red[threadIdx.x >> 6] = speed;
+    __syncthreads();
+    if (threadIdx.x == 0 && ctl) {
+      const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      atomicMax(&ctl->s[a.parity ^ 1].lambda,
+                static_cast<unsigned long long>(__double_as_longlong(mx)));
+    }
+  }
+  stage_epilogue(ctl, a.parity, a.stage, dt, leader);
+}
+
+template <int P, bool BF16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
+void euler_stage_mx(EulerMesh mesh, StageArgs a, const float *mxtab) {
+  __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
+  __shared__ double red[4];
+  mx_stage<P, BF16>(mesh, a, mxtab, xcd_block(), blockIdx.x == 0, tab, red);
+}
+
+// ------------------------------------------------------------ batching ----
+//
+// Several resident cases of one class (equation, order, precision) advance in
+// one launch per stage: a tile table lists (case, block) pairs, each case
+// carries its own buffers and step control, so every case takes its own CFL
+// dt and stops at its own end time. A tile computes exactly what the
+// single-case kernel computes for that block, so a case's result does not
+// depend on which other cases share the launch.
+struct BatchCase {
+  EulerMesh mesh;              // 2D Euler geometry and edge records
+  double *u, *w1, *w2;         // state and the two stage buffers
+  RunCtl *ctl;
+  int *list, *counters;        // troubled elements (limiting)
+  int sensor;
+  double threshold;
+  int highest_begin;
+};
+
+struct BatchStage {
+  int stage, parity;
+};
+
+__device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b) {
+  StageArgs a{};
+  const double ca[3] = {0.0, 0.75, 1.0 / 3.0};
+  const double cb[3] = {1.0, 0.25, 2.0 / 3.0};
+  a.uin = b.stage == 0 ? c.u : b.stage == 1 ? c.w1 : c.w2;
+  a.out = b.stage == 0 ? c.w1 : b.stage == 1 ? c.w2 : c.u;
+  a.un = c.u;
+  a.ca = ca[b.stage];
+  a.cb = cb[b.stage];
+  a.ctl = c.ctl;
+  a.parity = b.parity;
+  a.stage = b.stage;
+  a.want_lambda = b.stage == 2 ? 1 : 0;
+  a.sensor = c.sensor;
+  a.threshold = c.threshold;
+  a.highest_begin = c.highest_begin;
+  a.list = c.list;
+  a.counters = c.counters;
+  return a;
+}
+
+template <int P, bool BF16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
+void euler_stage_mx_batch(const BatchCase *cases, const int2 *tiles, BatchStage b,
+                          const float *mxtab) {
+  __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
+  __shared__ double red[4];
+  const int2 t = tiles[xcd_block()];
+  const BatchCase &c = cases[t.x];
+  mx_stage<P, BF16>(c.mesh, batch_args(c, b), mxtab, t.y, t.y == 0, tab, red);
 }
 
 // ------------------------------------------------- 1D scalar advection ----
@@ -860,18 +1325,19 @@ __device__ __forceinline__ double upwind(double a, double l, double r) {
 // One lane per element, state U[element][mode]. Face fluxes are those of the
 // reference's face loop (dg/scalar_advection/line_residual.hpp:76-103) with
 // exteriors from make_line_boundary_exterior (line_boundary_conditions.hpp:36-57).
-template <int P> __global__ __launch_bounds__(256) void line_stage(LineMeshDev mesh, StageArgs a) {
+template <int P>
+__device__ __forceinline__ void line_body(const LineMeshDev mesh, const StageArgs a, int block,
+                                          bool leader, double *tab) {
   using D = Line<P>;
   constexpr int Np = D::Np, NQ = D::NQ;
-  __shared__ double tab[D::kTable];
   double dt = a.dt;
-  if (!stage_prologue(a.ctl, a.parity, a.stage, dt))
+  if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
     return;
   for (int i = threadIdx.x; i < D::kTable; i += blockDim.x)
     tab[i] = mesh.table[i];
   __syncthreads();
   const int E = mesh.E;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = block * blockDim.x + threadIdx.x;
   if (e < E) {
     const double sa = mesh.wavespeed;
     double c[Np], acc[Np];
@@ -942,7 +1408,35 @@ template <int P> __global__ __launch_bounds__(256) void line_stage(LineMeshDev m
       a.out[e * Np + m] = y;
     }
   }
-  stage_epilogue(a.ctl, a.parity, a.stage, dt);
+  stage_epilogue(a.ctl, a.parity, a.stage, dt, leader);
+}
+
+template <int P> __global__ __launch_bounds__(256) void line_stage(LineMeshDev mesh, StageArgs a) {
+  __shared__ double tab[Line<P>::kTable];
+  line_body<P>(mesh, a, blockIdx.x, blockIdx.x == 0, tab);
+}
+
+// Batched 1D advection: (case, block) tiles as for the Euler classes.
+struct BatchLine {
+  LineMeshDev mesh;
+  double *u, *w1, *w2;
+  RunCtl *ctl;
+};
+
+template <int P>
+__global__ __launch_bounds__(256) void line_stage_batch(const BatchLine *cases, const int2 *tiles,
+                                                        BatchStage b) {
+  __shared__ double tab[Line<P>::kTable];
+  const int2 t = tiles[blockIdx.x];
+  const BatchLine &c = cases[t.x];
+  BatchCase shim{};
+  shim.u = c.u;
+  shim.w1 = c.w1;
+  shim.w2 = c.w2;
+  shim.ctl = c.ctl;
+  StageArgs a = batch_args(shim, b);
+  a.want_lambda = 0;
+  line_body<P>(c.mesh, a, t.y, t.y == 0, tab);
 }
 
 // ------------------------------------------------------- reductions ----
@@ -1111,13 +1605,12 @@ __device__ __forceinline__ double pressure_of(double g, double r, double mx, dou
 // euler_characteristic_weno.hpp:17-67, simple_weno.hpp:21-48,
 // euler_positivity.hpp:63-102). Results go to side[idx] for scatter().
 template <int P>
-__global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double *U, double *side,
-                                                     const int *list, const int *count,
-                                                     RunCtl *ctl, int parity, int *err) {
+__device__ __forceinline__ void limit_body(const LimiterDev L, const double *U, double *side,
+                                           const int *list, const int *count, RunCtl *ctl,
+                                           int parity, int *err, int idx) {
   constexpr int Np = Tri<P>::Np, NC = Tri<P>::NC;
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= *count)
     return;
   const int e = list[idx];
@@ -1311,13 +1804,11 @@ __global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double 
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
-                                                       const int *list, const int *count,
-                                                       RunCtl *ctl, int parity) {
+__device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
+                                             const int *count, RunCtl *ctl, int parity, int idx) {
   constexpr int Np = Tri<P>::Np;
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= *count)
     return;
   const int e = list[idx];
@@ -1327,6 +1818,53 @@ __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const d
 #pragma unroll
     for (int m = 0; m < Np; ++m)
       U[(v * Np + m) * E + e] = src[v * Np + m];
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double *U, double *side,
+                                                     const int *list, const int *count,
+                                                     RunCtl *ctl, int parity, int *err) {
+  limit_body<P>(L, U, side, list, count, ctl, parity, err, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
+                                                       const int *list, const int *count,
+                                                       RunCtl *ctl, int parity) {
+  scatter_body<P>(U, E, side, list, count, ctl, parity, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Batched limiter: tiles of 64 troubled-list slots per (case, chunk); the
+// stage's output buffer is limited in place as in the single-case path.
+struct BatchLimit {
+  LimiterDev L;
+  double *u, *w1, *w2;
+  double *side;
+  int *list, *counters;
+  RunCtl *ctl;
+  int *err;
+};
+
+__device__ __forceinline__ double *batch_out(const BatchLimit &c, int stage) {
+  return stage == 0 ? c.w1 : stage == 1 ? c.w2 : c.u;
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void limit_troubled_batch(const BatchLimit *cases,
+                                                           const int2 *tiles, BatchStage b) {
+  const int2 t = tiles[blockIdx.x];
+  const BatchLimit &c = cases[t.x];
+  limit_body<P>(c.L, batch_out(c, b.stage), c.side, c.list, c.counters + b.stage, c.ctl, b.parity,
+                c.err, t.y * 64 + threadIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void scatter_limited_batch(const BatchLimit *cases,
+                                                            const int2 *tiles, BatchStage b) {
+  const int2 t = tiles[blockIdx.x];
+  const BatchLimit &c = cases[t.x];
+  scatter_body<P>(batch_out(c, b.stage), c.L.E, c.side, c.list, c.counters + b.stage, c.ctl,
+                  b.parity, t.y * 64 + threadIdx.x);
 }
 
 }  // namespace whip

@@ -1,0 +1,691 @@
+// Serving: many resident cases advanced by batched stage kernels
+// (include/wasl_hip_serve.h). Part of the engine's translation unit: it is
+// included at the end of engine.hip and uses the engine's internals.
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <type_traits>
+
+#include "../../include/wasl_hip_serve.h"
+
+namespace {
+
+enum class Phase { queued, preparing, ready, resident, done };
+
+struct Request {
+  uint64_t id = 0;
+  whip::Mesh mesh;
+  wasl_hip_numerics num{};
+  int ic = 0;
+  std::vector<double> params;
+  double cfl = 0.0, end_time = 0.0;
+  uint64_t end_steps = 0, max_steps = 0;
+  Phase phase = Phase::queued;
+  size_t reserved = 0;                 // device bytes held against the budget
+  std::unique_ptr<wasl_hip_engine> eng;
+  // outcome
+  int status = WASL_HIP_OK;
+  std::string message;
+  std::vector<double> result;
+  double time = 0.0;
+  uint64_t step = 0;
+};
+
+// Device bytes of an engine for this case before it exists (state, stage
+// buffers, edge records, limiter stencil), for admission.
+size_t estimate_bytes(const whip::Mesh &m, const wasl_hip_numerics &n) {
+  const size_t E = m.elements(), p = static_cast<size_t>(n.polynomial_order);
+  if (m.dimension == 1)
+    return E * (3 * (p + 1) * 8 + 8) + (1 << 16);
+  const size_t Np = (p + 1) * (p + 2) / 2;
+  size_t b = E * (3 * 4 * Np * 8 + 5 * 8 + 3 * 4 * 2 + 9 * 8 + 8);
+  if (n.limiting && p > 0)
+    b += E * (3 * 4 + 3 * Np * Np * 8 + 2 * 8 + 4 + 1 + 4 * Np * 8);
+  return b + (1 << 20);
+}
+
+// Resident cases of one batched launch class.
+struct ClassKey {
+  int dim, p;
+  bool operator<(const ClassKey &o) const { return dim != o.dim ? dim < o.dim : p < o.p; }
+};
+
+}  // namespace
+
+struct wasl_hip_server {
+  wasl_hip_server_config cfg{};
+  hipStream_t stream = nullptr;
+  size_t budget = 0, reserved = 0;
+  uint64_t next_id = 1;
+  std::map<uint64_t, std::unique_ptr<Request>> requests;
+  std::deque<uint64_t> queue;        // waiting for a preparation slot
+  std::vector<uint64_t> resident;    // in admission order
+  std::vector<uint64_t> finished;    // done and unclaimed
+  bool dirty = true;
+  int64_t graphs = 0, rounds = 0, changes = 0;
+
+  // device tables of the resident set
+  Buffer euler_cases, euler_tiles, line_cases, line_tiles, limit_cases, limit_tiles, ctl_ptrs,
+      ctl_out, err_out;
+  struct Launch {
+    int kind;       // 0 Euler stage, 1 line stage, 2 limiter (troubled + scatter)
+    int p;
+    int tiles;
+    size_t tile_offset;   // into the kind's tile buffer
+  };
+  std::vector<Launch> launches;
+  std::vector<float> mx_host[4];
+  Buffer mxtab[4];
+  std::vector<whip::RunCtl *> ctl_list;
+  std::vector<uint64_t> ctl_ids;          // request of each ctl_list entry
+  whip::StepSlot *slots_host = nullptr;   // pinned
+  int *errs_host = nullptr;               // pinned
+  size_t slots_capacity = 0;
+  hipGraphExec_t graph = nullptr;
+
+  // workers
+  std::vector<std::thread> workers;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_ready;
+  std::deque<Request *> work;
+  bool stopping = false;
+
+  ~wasl_hip_server() {
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      stopping = true;
+    }
+    cv_work.notify_all();
+    for (auto &t : workers)
+      t.join();
+    if (graph)
+      (void)hipGraphExecDestroy(graph);
+    if (slots_host)
+      (void)hipHostFree(slots_host);
+    if (errs_host)
+      (void)hipHostFree(errs_host);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Worker: build the engine, project the initial condition, fix the end time.
+void prepare(Request &r) {
+  try {
+    auto g = std::make_unique<wasl_hip_engine>();
+    create_engine(r.mesh, r.num, *g);
+    const std::vector<double> U =
+        whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, r.ic, r.params.data(),
+                      static_cast<int64_t>(r.params.size()));
+    HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
+                            g->stream));
+    if (r.end_steps > 0) {
+      const double lambda = resident_speed(*g);
+      const double stable = lambda > 0.0
+                                ? r.cfl * g->hmin / (lambda * static_cast<double>(2 * g->p + 1))
+                                : 0.0;
+      if (!(stable > 0.0))
+        throw SimulationError("simulation: timestep cannot advance time");
+      r.end_time = static_cast<double>(r.end_steps) * stable;
+    }
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    r.eng = std::move(g);
+  } catch (const SimulationError &ex) {
+    r.status = WASL_HIP_SIMULATION;
+    r.message = ex.what();
+  } catch (const DeviceError &ex) {
+    r.status = WASL_HIP_DEVICE;
+    r.message = ex.what();
+  } catch (const std::exception &ex) {
+    r.status = WASL_HIP_ERROR;
+    r.message = ex.what();
+  }
+}
+
+void worker_loop(wasl_hip_server *s) {
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lock(s->mu);
+      s->cv_work.wait(lock, [&] { return s->stopping || !s->work.empty(); });
+      if (s->stopping)
+        return;
+      r = s->work.front();
+      s->work.pop_front();
+    }
+    prepare(*r);
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      r->phase = r->status == WASL_HIP_OK ? Phase::ready : Phase::done;
+    }
+    s->cv_ready.notify_all();
+  }
+}
+
+// Step control of a newly admitted case, at the server's step parity 0
+// (every round is an even number of steps).
+void admit_ctl(wasl_hip_server &s, Request &r) {
+  wasl_hip_engine &g = *r.eng;
+  whip::RunCtl h{};
+  h.end_time = r.end_time;
+  h.cfl_length = r.cfl * g.hmin;
+  h.order_factor = static_cast<double>(2 * g.p + 1);
+  h.max_steps = r.max_steps;
+  h.fixed_speed = g.dim == 1 ? 1 : 0;
+  if (g.dim == 1) {
+    const double a = std::fabs(g.num.wavespeed);
+    std::memcpy(&h.s[0].lambda, &a, sizeof(double));
+  }
+  HIPCHECK(hipMemcpyAsync(g.ctl.p, &h, sizeof h, hipMemcpyHostToDevice, s.stream));
+  if (g.dim == 2)
+    hipLaunchKernelGGL(whip::cfl_speed, dim3((g.E + 255) / 256), dim3(256), 0, s.stream,
+                       g.u.as<double>(), g.E, g.Np, g.num.heat_capacity_ratio,
+                       g.ctl.as<whip::RunCtl>(), 0);
+  HIPCHECK(hipGetLastError());
+  // the copy above reads the host struct: complete it before it goes out of scope
+  HIPCHECK(hipStreamSynchronize(s.stream));
+}
+
+template <typename T> void put_table(Buffer &b, const std::vector<T> &v, hipStream_t st) {
+  const size_t n = v.size() * sizeof(T);
+  if (b.bytes < n) {
+    size_t unused = 0;
+    b.alloc(n, unused);
+  }
+  if (n)
+    HIPCHECK(hipMemcpyAsync(b.p, v.data(), n, hipMemcpyHostToDevice, st));
+}
+
+// Batched launch tables of the resident set.
+void rebuild(wasl_hip_server &s) {
+  std::map<ClassKey, std::vector<Request *>> cls;
+  for (uint64_t id : s.resident) {
+    Request *r = s.requests.at(id).get();
+    cls[{r->eng->dim, r->eng->p}].push_back(r);
+  }
+  std::vector<whip::BatchCase> ec;
+  std::vector<whip::BatchLine> lc;
+  std::vector<whip::BatchLimit> mc;
+  std::vector<int2> et, lt, mt;
+  s.launches.clear();
+  s.ctl_list.clear();
+  s.ctl_ids.clear();
+  for (auto &[key, list] : cls) {
+    if (key.dim == 2) {
+      const size_t off = et.size();
+      std::vector<Request *> limited;
+      for (Request *r : list) {
+        wasl_hip_engine &g = *r->eng;
+        whip::BatchCase c{};
+        c.mesh = g.euler_mesh();
+        c.u = g.u.as<double>();
+        c.w1 = g.w1.as<double>();
+        c.w2 = g.w2.as<double>();
+        c.ctl = g.ctl.as<whip::RunCtl>();
+        const bool lim = limiting(g);
+        c.sensor = lim ? 1 : 0;
+        c.threshold = g.num.sensor_threshold;
+        c.highest_begin = g.stencil.highest_begin;
+        c.list = g.list.as<int>();
+        c.counters = g.counters.as<int>();
+        const int idx = static_cast<int>(ec.size());
+        ec.push_back(c);
+        for (int b = 0; b < (g.E + 63) / 64; ++b)
+          et.push_back(make_int2(idx, b));
+        if (lim)
+          limited.push_back(r);
+        s.ctl_list.push_back(c.ctl);
+        s.ctl_ids.push_back(r->id);
+      }
+      s.launches.push_back({0, key.p, static_cast<int>(et.size() - off), off});
+      if (!limited.empty()) {
+        const size_t moff = mt.size();
+        for (Request *r : limited) {
+          wasl_hip_engine &g = *r->eng;
+          whip::BatchLimit m{};
+          m.L = g.limiter();
+          m.u = g.u.as<double>();
+          m.w1 = g.w1.as<double>();
+          m.w2 = g.w2.as<double>();
+          m.side = g.side.as<double>();
+          m.list = g.list.as<int>();
+          m.counters = g.counters.as<int>();
+          m.ctl = g.ctl.as<whip::RunCtl>();
+          m.err = g.err.as<int>();
+          const int idx = static_cast<int>(mc.size());
+          mc.push_back(m);
+          for (int b = 0; b < (g.E + 63) / 64; ++b)
+            mt.push_back(make_int2(idx, b));
+        }
+        s.launches.push_back({2, key.p, static_cast<int>(mt.size() - moff), moff});
+      }
+    } else {
+      const size_t off = lt.size();
+      for (Request *r : list) {
+        wasl_hip_engine &g = *r->eng;
+        whip::BatchLine c{};
+        c.mesh = g.line_mesh();
+        c.u = g.u.as<double>();
+        c.w1 = g.w1.as<double>();
+        c.w2 = g.w2.as<double>();
+        c.ctl = g.ctl.as<whip::RunCtl>();
+        const int idx = static_cast<int>(lc.size());
+        lc.push_back(c);
+        for (int b = 0; b < (g.E + 255) / 256; ++b)
+          lt.push_back(make_int2(idx, b));
+        s.ctl_list.push_back(c.ctl);
+        s.ctl_ids.push_back(r->id);
+      }
+      s.launches.push_back({1, key.p, static_cast<int>(lt.size() - off), off});
+    }
+  }
+  put_table(s.euler_cases, ec, s.stream);
+  put_table(s.euler_tiles, et, s.stream);
+  put_table(s.line_cases, lc, s.stream);
+  put_table(s.line_tiles, lt, s.stream);
+  put_table(s.limit_cases, mc, s.stream);
+  put_table(s.limit_tiles, mt, s.stream);
+  put_table(s.ctl_ptrs, s.ctl_list, s.stream);
+  const size_t n = s.ctl_list.size();
+  if (s.slots_capacity < n) {
+    if (s.slots_host)
+      HIPCHECK(hipHostFree(s.slots_host));
+    if (s.errs_host)
+      HIPCHECK(hipHostFree(s.errs_host));
+    s.slots_host = nullptr;
+    s.errs_host = nullptr;
+    s.slots_capacity = std::max<size_t>(2 * n, 64);
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&s.slots_host),
+                           s.slots_capacity * sizeof(whip::StepSlot)));
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&s.errs_host),
+                           s.slots_capacity * sizeof(int)));
+    size_t unused = 0;
+    s.ctl_out.alloc(s.slots_capacity * sizeof(whip::StepSlot), unused);
+    s.err_out.alloc(s.slots_capacity * sizeof(int), unused);
+  }
+  HIPCHECK(hipStreamSynchronize(s.stream));
+}
+
+template <int P> void launch_euler_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
+                                         whip::BatchStage b) {
+  const whip::BatchCase *cases = s.euler_cases.as<whip::BatchCase>();
+  const int2 *tiles = s.euler_tiles.as<int2>() + l.tile_offset;
+  const float *t = s.mxtab[P].as<float>();
+  if (s.cfg.precision == WASL_HIP_BF16)
+    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, true>), dim3(l.tiles), dim3(256), 0,
+                       s.stream, cases, tiles, b, t);
+  else
+    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, false>), dim3(l.tiles), dim3(256), 0,
+                       s.stream, cases, tiles, b, t);
+}
+
+template <int P> void launch_line_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
+                                        whip::BatchStage b) {
+  hipLaunchKernelGGL((whip::line_stage_batch<P>), dim3(l.tiles), dim3(256), 0, s.stream,
+                     s.line_cases.as<whip::BatchLine>(), s.line_tiles.as<int2>() + l.tile_offset,
+                     b);
+}
+
+template <int P> void launch_limit_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
+                                         whip::BatchStage b) {
+  const whip::BatchLimit *cases = s.limit_cases.as<whip::BatchLimit>();
+  const int2 *tiles = s.limit_tiles.as<int2>() + l.tile_offset;
+  hipLaunchKernelGGL((whip::limit_troubled_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream, cases,
+                     tiles, b);
+  hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream,
+                     cases, tiles, b);
+}
+
+template <typename F> void by_order(int p, F &&f) {
+  switch (p) {
+  case 0: f(std::integral_constant<int, 0>{}); break;
+  case 1: f(std::integral_constant<int, 1>{}); break;
+  case 2: f(std::integral_constant<int, 2>{}); break;
+  default: f(std::integral_constant<int, 3>{}); break;
+  }
+}
+
+// One round's launches: steps_per_batch steps of three stages; per stage the
+// Euler classes, then their limiters, then the line classes.
+void launch_round(wasl_hip_server &s) {
+  int parity = 0;
+  for (int k = 0; k < s.cfg.steps_per_batch; ++k) {
+    for (int stage = 0; stage < 3; ++stage) {
+      const whip::BatchStage b{stage, parity};
+      for (const auto &l : s.launches) {
+        if (l.tiles == 0)
+          continue;
+        by_order(l.p, [&](auto P) {
+          if (l.kind == 0)
+            launch_euler_class<decltype(P)::value>(s, l, b);
+          else if (l.kind == 1)
+            launch_line_class<decltype(P)::value>(s, l, b);
+          else if (decltype(P)::value > 0)
+            launch_limit_class<decltype(P)::value>(s, l, b);
+        });
+      }
+    }
+    parity ^= 1;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+__global__ void gather_slots(whip::RunCtl *const *ctls, int n, whip::StepSlot *out, int *errs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[i] = ctls[i]->s[0];
+    errs[i] = ctls[i]->err;
+  }
+}
+
+void retire(wasl_hip_server &s, Request &r, int status, const std::string &message) {
+  wasl_hip_engine &g = *r.eng;
+  r.result.resize(g.state_size());
+  HIPCHECK(hipMemcpy(r.result.data(), g.u.p, r.result.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  r.status = status;
+  r.message = message;
+  r.eng.reset();
+  s.reserved -= r.reserved;
+  r.reserved = 0;
+  r.phase = Phase::done;
+}
+
+}  // namespace
+
+extern "C" {
+
+void wasl_hip_server_config_default(wasl_hip_server_config *c) {
+  if (!c)
+    return;
+  *c = wasl_hip_server_config{};
+  c->precision = WASL_HIP_MIXED;
+  c->max_resident = 256;
+  c->memory_budget = 0;
+  c->steps_per_batch = 16;
+  c->use_graphs = 1;
+  c->setup_threads = 4;
+}
+
+int wasl_hip_server_create(const wasl_hip_server_config *config, wasl_hip_server **out) {
+  return guarded([&] {
+    if (!out)
+      throw std::runtime_error("server: null argument");
+    auto s = std::make_unique<wasl_hip_server>();
+    if (config)
+      s->cfg = *config;
+    else
+      wasl_hip_server_config_default(&s->cfg);
+    wasl_hip_server_config &c = s->cfg;
+    if (c.precision != WASL_HIP_MIXED && c.precision != WASL_HIP_BF16)
+      throw std::runtime_error("server: precision must be MIXED or BF16");
+    if (c.steps_per_batch <= 0 || c.steps_per_batch % 2 != 0)
+      throw std::runtime_error("server: steps_per_batch must be even and positive");
+    if (c.max_resident <= 0)
+      throw std::runtime_error("server: max_resident must be positive");
+    c.setup_threads = std::max(1, std::min(16, c.setup_threads));
+    HIPCHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    s->budget = c.memory_budget ? static_cast<size_t>(c.memory_budget)
+                                : static_cast<size_t>(0.8 * static_cast<double>(free_b));
+    for (int p = 0; p <= 3; ++p) {
+      s->mx_host[p] = mixed_tables(p);
+      size_t unused = 0;
+      s->mxtab[p].alloc(s->mx_host[p].size() * sizeof(float), unused);
+      HIPCHECK(hipMemcpyAsync(s->mxtab[p].p, s->mx_host[p].data(),
+                              s->mx_host[p].size() * sizeof(float), hipMemcpyHostToDevice,
+                              s->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < c.setup_threads; ++i)
+      s->workers.emplace_back(worker_loop, s.get());
+    *out = s.release();
+  });
+}
+
+void wasl_hip_server_destroy(wasl_hip_server *s) { delete s; }
+
+int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
+                           const wasl_hip_numerics *numerics, int ic, const double *params,
+                           int64_t param_count, double cfl, double end_time, uint64_t end_steps,
+                           uint64_t max_steps, uint64_t *id) {
+  return guarded([&] {
+    if (!s || !mesh || !numerics || !id || (param_count > 0 && !params))
+      throw std::runtime_error("server: null argument");
+    if (!(cfl > 0.0) || (end_steps == 0 && !(end_time > 0.0)) || max_steps == 0)
+      throw std::runtime_error("server: invalid run control");
+    auto r = std::make_unique<Request>();
+    r->mesh = mesh->m;
+    r->num = *numerics;
+    if (r->mesh.dimension == 2)
+      r->num.precision = s->cfg.precision;
+    else
+      r->num.precision = WASL_HIP_FP64;
+    r->ic = ic;
+    r->params.assign(params, params + std::max<int64_t>(param_count, 0));
+    r->cfl = cfl;
+    r->end_time = end_time;
+    r->end_steps = end_steps;
+    r->max_steps = max_steps;
+    r->id = s->next_id++;
+    *id = r->id;
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->queue.push_back(r->id);
+    s->requests[r->id] = std::move(r);
+  });
+}
+
+int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capacity,
+                         int64_t *count) {
+  return guarded([&] {
+    if (!s)
+      throw std::runtime_error("server: null argument");
+    std::vector<uint64_t> done_now, ready_now;
+    {
+      std::unique_lock<std::mutex> lock(s->mu);
+      // start preparations in arrival order while the budget and slots allow
+      size_t active = s->resident.size();
+      for (auto &[id, r] : s->requests)
+        if (r->phase == Phase::preparing || r->phase == Phase::ready)
+          ++active;
+      while (!s->queue.empty() && active < static_cast<size_t>(s->cfg.max_resident)) {
+        Request &r = *s->requests.at(s->queue.front());
+        const size_t need = estimate_bytes(r.mesh, r.num);
+        if (s->reserved + need > s->budget && s->reserved > 0)
+          break;
+        s->queue.pop_front();
+        r.reserved = need;
+        s->reserved += need;
+        r.phase = Phase::preparing;
+        s->work.push_back(&r);
+        ++active;
+      }
+      s->cv_work.notify_all();
+      if (s->resident.empty()) {
+        const auto has_ready = [&] {
+          for (auto &[id, r] : s->requests)
+            if (r->phase == Phase::ready || (r->phase == Phase::done && r->eng == nullptr &&
+                                              r->status != WASL_HIP_OK && r->reserved))
+              return true;
+          return false;
+        };
+        s->cv_ready.wait_for(lock, std::chrono::milliseconds(50), has_ready);
+      }
+      // failed preparations finish here; ready ones are admitted
+      for (auto &[id, r] : s->requests) {
+        if (r->phase == Phase::done && r->reserved && !r->eng) {
+          s->reserved -= r->reserved;
+          r->reserved = 0;
+          done_now.push_back(id);
+        } else if (r->phase == Phase::ready) {
+          r->phase = Phase::resident;
+          ready_now.push_back(id);
+        }
+      }
+    }
+    for (uint64_t id : ready_now) {
+      Request *r = s->requests.at(id).get();
+      // replace the estimate by what the engine holds
+      s->reserved = s->reserved - r->reserved + r->eng->device_bytes;
+      r->reserved = r->eng->device_bytes;
+      admit_ctl(*s, *r);
+      s->resident.push_back(id);
+      s->dirty = true;
+    }
+    if (!s->resident.empty()) {
+      if (s->dirty) {
+        if (s->graph) {
+          HIPCHECK(hipGraphExecDestroy(s->graph));
+          s->graph = nullptr;
+        }
+        rebuild(*s);
+        ++s->changes;
+        s->dirty = false;
+      }
+      if (s->cfg.use_graphs) {
+        if (!s->graph) {
+          hipGraph_t gr;
+          HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+          launch_round(*s);
+          HIPCHECK(hipStreamEndCapture(s->stream, &gr));
+          HIPCHECK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
+          HIPCHECK(hipGraphDestroy(gr));
+          ++s->graphs;
+        }
+        HIPCHECK(hipGraphLaunch(s->graph, s->stream));
+      } else {
+        launch_round(*s);
+      }
+      ++s->rounds;
+      // gather the step slots (parity 0 after an even round) and error codes,
+      // in the order of the launch tables
+      const int n = static_cast<int>(s->ctl_list.size());
+      hipLaunchKernelGGL(gather_slots, dim3((n + 63) / 64), dim3(64), 0, s->stream,
+                         s->ctl_ptrs.as<whip::RunCtl *>(), n, s->ctl_out.as<whip::StepSlot>(),
+                         s->err_out.as<int>());
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(s->slots_host, s->ctl_out.p, n * sizeof(whip::StepSlot),
+                              hipMemcpyDeviceToHost, s->stream));
+      HIPCHECK(hipMemcpyAsync(s->errs_host, s->err_out.p, n * sizeof(int), hipMemcpyDeviceToHost,
+                              s->stream));
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      std::vector<uint64_t> still;
+      for (int i = 0; i < n; ++i) {
+        Request &r = *s->requests.at(s->ctl_ids[i]);
+        const whip::StepSlot &st = s->slots_host[i];
+        r.time = st.time;
+        r.step = st.step;
+        if (s->errs_host[i]) {
+          retire(*s, r, WASL_HIP_SIMULATION, device_error_text(s->errs_host[i]));
+          done_now.push_back(r.id);
+        } else if (!(st.time < r.end_time) || st.step >= r.max_steps) {
+          const bool short_of_end = st.time < r.end_time;
+          retire(*s, r, short_of_end ? WASL_HIP_SIMULATION : WASL_HIP_OK,
+                 short_of_end ? "simulation: max_steps reached before end_time" : "");
+          done_now.push_back(r.id);
+        } else {
+          still.push_back(r.id);
+        }
+      }
+      if (still.size() != s->resident.size()) {
+        // keep admission order among the remaining cases
+        std::vector<uint64_t> keep;
+        for (uint64_t id : s->resident)
+          if (s->requests.at(id)->phase == Phase::resident)
+            keep.push_back(id);
+        s->resident.swap(keep);
+        s->dirty = true;
+      }
+    }
+    for (uint64_t id : done_now)
+      s->finished.push_back(id);
+    int64_t k = 0;
+    while (k < capacity && !s->finished.empty()) {
+      if (completed)
+        completed[k] = s->finished.front();
+      s->finished.erase(s->finished.begin());
+      ++k;
+    }
+    if (count)
+      *count = k;
+  });
+}
+
+int wasl_hip_server_result(wasl_hip_server *s, uint64_t id, double *state, int64_t capacity,
+                           double *time, uint64_t *step, int *case_status) {
+  return guarded([&] {
+    if (!s)
+      throw std::runtime_error("server: null argument");
+    auto it = s->requests.find(id);
+    if (it == s->requests.end() || it->second->phase != Phase::done)
+      throw std::runtime_error("server: request is not finished");
+    Request &r = *it->second;
+    if (state) {
+      if (capacity < static_cast<int64_t>(r.result.size()))
+        throw std::runtime_error("server: state buffer too small");
+      std::copy(r.result.begin(), r.result.end(), state);
+    }
+    if (time)
+      *time = r.time;
+    if (step)
+      *step = r.step;
+    if (case_status)
+      *case_status = r.status;
+    if (r.status != WASL_HIP_OK)
+      g_error = r.message;
+    s->requests.erase(it);
+  });
+}
+
+int64_t wasl_hip_server_state_size(const wasl_hip_server *s, uint64_t id) {
+  if (!s)
+    return -1;
+  auto it = s->requests.find(id);
+  if (it == s->requests.end())
+    return -1;
+  const Request &r = *it->second;
+  if (!r.result.empty())
+    return static_cast<int64_t>(r.result.size());
+  if (r.eng)
+    return static_cast<int64_t>(r.eng->state_size());
+  return -1;
+}
+
+int64_t wasl_hip_server_count(const wasl_hip_server *s, int what) {
+  if (!s)
+    return -1;
+  int64_t n = 0;
+  switch (what) {
+  case 0:
+    return static_cast<int64_t>(s->queue.size());
+  case 1:
+    for (auto &[id, r] : s->requests)
+      if (r->phase == Phase::preparing || r->phase == Phase::ready)
+        ++n;
+    return n;
+  case 2:
+    return static_cast<int64_t>(s->resident.size());
+  case 3:
+    return static_cast<int64_t>(s->finished.size());
+  case 4:
+    return static_cast<int64_t>(s->reserved);
+  case 5:
+    return s->graphs;
+  case 6:
+    return s->rounds;
+  case 7:
+    return s->changes;
+  default:
+    return -1;
+  }
+}
+
+}  // extern "C"

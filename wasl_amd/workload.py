@@ -1,0 +1,181 @@
+"""The serving mix and its two executions, plain and served.
+
+The mix: seeded requests, 70 % 2D Euler and 30 % 1D advection; triangle
+counts log-uniform in [512, 131072] (quad-split n x n grids, n = round(sqrt(N/2)))
+and line cells log-uniform in [256, 65536]; order uniform in P1-P3; Euler
+initial states drawn from {isentropic vortex, radial double rarefaction,
+smooth Fourier field}; each run lasts a uniform 200-2000 steps of its initial
+CFL step. The rarefaction runs on [-1,1]^2 with transmissive boundaries,
+characteristic WENO and positivity limiting (floors 1e-13, CFL 0.1); vortex
+and smooth fields on periodic [0,10]^2 at CFL 0.15; advection of the pulse on
+periodic [0,1] at CFL 0.1.
+
+plain:  one case at a time through the FP64 engine (create, project, advance)
+served: all cases through a Server (mixed precision, continuous batching)
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine as W
+from . import meshgen as MG
+
+
+@dataclass
+class Request:
+    index: int
+    kind: str            # "vortex", "smooth", "rarefaction", "advection"
+    size: int            # triangles or cells
+    order: int
+    steps: int
+    mesh_key: tuple
+    equation: int
+    ic: int
+    params: tuple
+    cfl: float
+    boundary: tuple = (W.PERIODIC, W.PERIODIC)
+    limiting: bool = False
+    floors: float = 1e-10
+    # filled by the plain run
+    end_time: float = 0.0
+    dofs: int = 0
+    plain_seconds: float = 0.0
+    plain_steps: int = 0
+    plain_state: np.ndarray | None = field(default=None, repr=False)
+
+
+def serving_mix(count: int, seed: int = 0x5EED) -> list[Request]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        order = int(rng.integers(1, 4))
+        steps = int(rng.integers(200, 2001))
+        if rng.random() < 0.7:
+            tris = math.exp(rng.uniform(math.log(512), math.log(131072)))
+            n = max(2, int(round(math.sqrt(tris / 2))))
+            kind = ("vortex", "rarefaction", "smooth")[int(rng.integers(0, 3))]
+            if kind == "vortex":
+                out.append(Request(i, kind, 2 * n * n, order, steps, ("box10", n), W.EULER,
+                                   W.IC_VORTEX, (5.0, 5.0, 5.0), 0.15))
+            elif kind == "smooth":
+                prm = [10.0]
+                for _ in range(4):
+                    prm += [float(rng.uniform(-1, 1)), int(rng.integers(1, 4)),
+                            int(rng.integers(1, 4)), float(rng.uniform(0, 2 * math.pi))]
+                out.append(Request(i, kind, 2 * n * n, order, steps, ("box10", n), W.EULER,
+                                   W.IC_SMOOTH, tuple(prm), 0.15))
+            else:
+                out.append(Request(i, kind, 2 * n * n, order, steps, ("box1", n), W.EULER,
+                                   W.IC_RAREFACTION, (1.0, 0.4, 2.0), 0.1,
+                                   (W.OUTFLOW, W.OUTFLOW), True, 1e-13))
+        else:
+            cells = int(round(math.exp(rng.uniform(math.log(256), math.log(65536)))))
+            out.append(Request(i, "advection", cells, order, steps, ("line", cells),
+                               W.SCALAR_ADVECTION, W.IC_SCALAR_PULSE, (), 0.1))
+    return out
+
+
+class MeshCache:
+    def __init__(self):
+        self.meshes = {}
+
+    def get(self, key):
+        if key not in self.meshes:
+            kind, n = key
+            if kind == "box10":
+                m = MG.jittered_triangles(n, (0.0, 0.0), (10.0, 10.0), 0.15)
+            elif kind == "box1":
+                m = MG.jittered_triangles(n, (-1.0, -1.0), (1.0, 1.0), 0.15)
+            else:
+                m = MG.jittered_line(n)
+            self.meshes[key] = W.Mesh.from_input(m)
+        return self.meshes[key]
+
+
+def _engine(r: Request, mesh, precision=W.FP64):
+    return W.Engine(mesh, r.equation, r.order, boundary=r.boundary, limiting=r.limiting,
+                    density_floor=r.floors, pressure_floor=r.floors, precision=precision,
+                    wavespeed=1.0)
+
+
+def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True) -> float:
+    """One case at a time, FP64: engine setup, projection and the stepping
+    loop are the service time. Sets each request's end time (its steps times
+    the initial CFL step) and DOF count; returns the total seconds."""
+    import torch
+
+    total = 0.0
+    for r in reqs:
+        mesh = meshes.get(r.mesh_key)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = _engine(r, mesh)
+        g.project(r.ic, r.params)
+        r.end_time = r.steps * g.stable_time_step(r.cfl)
+        t, s, _ = g.advance(r.cfl, r.end_time)
+        torch.cuda.synchronize()
+        r.plain_seconds = time.perf_counter() - t0
+        r.plain_steps = s
+        r.dofs = g.E * g.Np * g.V
+        if keep_states:
+            r.plain_state = g.state
+        total += r.plain_seconds
+        del g
+    return total
+
+
+def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=W.MIXED,
+               steps_per_batch: int = 16, max_resident: int = 256):
+    """Submit each request at its arrival offset (all at once when arrivals is
+    None) and pump the server until every request is finished. Returns
+    (per-request (arrival, completion, steps, state) dict, makespan seconds)."""
+    from .serve import Server
+
+    s = Server(precision=precision, steps_per_batch=steps_per_batch, max_resident=max_resident)
+    order = sorted(range(len(reqs)), key=lambda k: 0.0 if arrivals is None else arrivals[k])
+    pending = list(order)
+    ids = {}
+    done = {}
+    t0 = time.perf_counter()
+    while pending or len(done) < len(reqs):
+        now = time.perf_counter() - t0
+        while pending and (arrivals is None or arrivals[pending[0]] <= now):
+            k = pending.pop(0)
+            r = reqs[k]
+            rid = s.submit(meshes.get(r.mesh_key), r.equation, r.order, r.ic, r.params, cfl=r.cfl,
+                           end_time=r.end_time, boundary=r.boundary, limiting=r.limiting,
+                           density_floor=r.floors, pressure_floor=r.floors, wavespeed=1.0)
+            ids[rid] = (k, now if arrivals is not None else 0.0)
+        if s.idle and not s.count(3):
+            if pending:
+                time.sleep(max(0.0, min(0.01, arrivals[pending[0]] - now)))
+            continue
+        for rid in s.pump():
+            k, arrived = ids[rid]
+            finish = time.perf_counter() - t0
+            state, t, steps = s.result(rid)
+            done[k] = (arrived, finish, steps, state)
+    makespan = time.perf_counter() - t0
+    s.close()
+    return done, makespan
+
+
+def fifo_times(arrivals, service):
+    """Time to solution of a single FIFO server with the given arrival
+    offsets and service times."""
+    free = 0.0
+    out = []
+    for a, s in sorted(zip(arrivals, service)):
+        start = max(a, free)
+        free = start + s
+        out.append(free - a)
+    return out
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs, np.float64), q))
