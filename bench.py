@@ -110,10 +110,19 @@ def serving_measurement(count: int, seed: int) -> dict:
     for r in mix:
         meshes.get(r.mesh_key)
     log(f"serving mix: {count} requests ({sum(r.kind != 'advection' for r in mix)} Euler)")
-    plain_total = WL.run_plain(mix, meshes)
+    WL.run_plain(mix, meshes)
+    failed = [r.index for r in mix if r.plain_error]
+    for r in mix:
+        if r.plain_error:
+            log(f"  plain stop: request {r.index} {r.kind} P{r.order} {r.size} elements, "
+                f"{r.steps} steps: {r.plain_error}")
+    mix = [r for r in mix if not r.plain_error]
+    count = len(mix)
+    plain_total = sum(r.plain_seconds for r in mix)
     updates = sum(r.dofs * r.plain_steps for r in mix)
     plain_rate = updates / plain_total
-    log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s")
+    log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s"
+        f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
 
     done, makespan = WL.run_served(mix, meshes)
     served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
@@ -129,17 +138,23 @@ def serving_measurement(count: int, seed: int) -> dict:
     tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
     plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
 
-    worst = 0.0
+    worst = {}
+    served_failed = sum(1 for v in done.values() if v[3] is None)
     for k, r in enumerate(mix):
+        if done[k][3] is None:
+            log(f"  served stop: request {r.index} {r.kind} P{r.order} {r.size} elements")
+            continue
         if r.kind == "advection":
             continue
         Np = (r.order + 1) * (r.order + 2) // 2
         E = r.size
         a = done[k][3].reshape(4, Np, E)[0]
         b = r.plain_state.reshape(4, Np, E)[0]
-        worst = max(worst, float(np.max(np.abs(a - b))))
+        worst[r.kind] = max(worst.get(r.kind, 0.0), float(np.max(np.abs(a - b))))
     out = {
         "requests": count,
+        "requests_failed_plain": len(failed),
+        "requests_failed_served": served_failed,
         "mix": "70% 2D Euler (vortex | radial rarefaction, limited | smooth Fourier), "
                "30% 1D advection; tris log-U[512,131072], cells log-U[256,65536], P U{1,2,3}, "
                "200-2000 steps",
@@ -155,10 +170,13 @@ def serving_measurement(count: int, seed: int) -> dict:
         "plain_tts_p99_s": WL.percentile(plain_tts, 99),
         "open_loop_makespan_s": makespan_o,
         "density_linf_served_vs_fp64": worst,
+        "density_linf_note": "max over requests of each kind of |rho coefficients, served "
+                             "(mixed) - plain (FP64)|; limited rarefactions can flip a "
+                             "troubled-cell decision between the two",
     }
     log(f"served open-loop at {rate:.2f} req/s: TTS p50 {out['tts_p50_s']:.3f} s "
         f"p99 {out['tts_p99_s']:.3f} s (plain p50 {out['plain_tts_p50_s']:.3f} s "
-        f"p99 {out['plain_tts_p99_s']:.3f} s); density L-inf vs FP64 {worst:.2e}")
+        f"p99 {out['plain_tts_p99_s']:.3f} s); density L-inf vs FP64 {worst}")
     return out
 
 

@@ -44,8 +44,10 @@ def test_mixed_residual_periodic(gpu, order):
         assert err <= tol
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [2, 3])
 def test_mixed_residual_transmissive(gpu, order):
+    # (the unlimited P1 projection of the rarefaction is unphysical at volume
+    # points, in the oracle as well; P1 is covered by the periodic test)
     m = K.rarefaction_mesh(10)
     o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
     # the radial rarefaction: outflow through every boundary face, O(1) residuals

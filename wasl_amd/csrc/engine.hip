@@ -8,8 +8,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -129,6 +132,8 @@ struct wasl_hip_engine {
   Buffer u, w1, w2;
   // geometry and tables
   Buffer geo, nbr, meta, fn, bdonor, bbasis, table, vtable, inv_jac, mxtable;
+  // mixed precision: FP32 edge traces of u, w1, w2 ([E][3][4][4] floats each)
+  Buffer tr_u, tr_w1, tr_w2;
   // limiter
   Buffer lnbr, T, nrm, check, flags, list, counters, side, err;
   // control and reductions
@@ -160,6 +165,21 @@ struct wasl_hip_engine {
   }
 
   size_t state_size() const { return static_cast<size_t>(V) * Np * E; }
+
+  bool mixed() const { return dim == 2 && num.precision != WASL_HIP_FP64; }
+
+  // The trace buffer that belongs to a state buffer (mixed precision only).
+  float *traces_of(const double *p) const {
+    if (!mixed())
+      return nullptr;
+    if (p == u.as<double>())
+      return tr_u.as<float>();
+    if (p == w1.as<double>())
+      return tr_w1.as<float>();
+    if (p == w2.as<double>())
+      return tr_w2.as<float>();
+    return nullptr;
+  }
 
   whip::EulerMesh euler_mesh() const {
     whip::EulerMesh m;
@@ -310,7 +330,7 @@ template <int P> std::vector<float> mixed_tables() {
     const int i = l & 15, kk = l >> 4;
     for (int tt = 0; tt < MT; ++tt) {
       for (int s = 0; s < KS; ++s)
-        t[X::kAvol + (tt * KS + s) * 64 + l] = static_cast<float>(vb(16 * tt + i, 4 * s + kk));
+        t[X::kAvol + (tt * KS + s) * 64 + l] = static_cast<float>(vb(16 * tt + i, 4 * kk + s));
       for (int r = 0; r < 4; ++r)
         for (int d = 0; d < 2; ++d)
           t[X::kAgrad + ((tt * 4 + r) * 2 + d) * 64 + l] =
@@ -327,7 +347,7 @@ template <int P> std::vector<float> mixed_tables() {
       std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 1) * 64 + l) * 2], he, sizeof he);
     }
     for (int s = 0; s < KS; ++s)
-      t[X::kAface + s * 64 + l] = static_cast<float>(eb(i >> 2, i & 3, 4 * s + kk));
+      t[X::kAface + s * 64 + l] = static_cast<float>(eb(i >> 2, i & 3, 4 * kk + s));
     for (int s = 0; s < NE; ++s)
       t[X::kAlift + s * 64 + l] = static_cast<float>(eb(kk, s, i));
   }
@@ -428,6 +448,27 @@ void launch_stage(const wasl_hip_engine &g, const whip::StageArgs &a) {
   HIPCHECK(hipGetLastError());
 }
 
+// FP32 edge traces of a state buffer (mixed precision), on `stream`.
+void launch_traces(const wasl_hip_engine &g, const double *U, float *T, const float *mxtab,
+                   hipStream_t stream) {
+  const dim3 grid((g.E + 63) / 64);
+  switch (g.p) {
+  case 0:
+    hipLaunchKernelGGL(whip::mx_traces<0>, grid, dim3(256), 0, stream, U, g.E, T, mxtab);
+    break;
+  case 1:
+    hipLaunchKernelGGL(whip::mx_traces<1>, grid, dim3(256), 0, stream, U, g.E, T, mxtab);
+    break;
+  case 2:
+    hipLaunchKernelGGL(whip::mx_traces<2>, grid, dim3(256), 0, stream, U, g.E, T, mxtab);
+    break;
+  default:
+    hipLaunchKernelGGL(whip::mx_traces<3>, grid, dim3(256), 0, stream, U, g.E, T, mxtab);
+    break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
 template <int P>
 void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip::RunCtl *ctl,
                     int parity) {
@@ -436,7 +477,8 @@ void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip:
   hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, g.stream, L, U,
                      g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.err.as<int>());
   hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, g.stream, U, g.E,
-                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity);
+                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.traces_of(U),
+                     g.mxtable.as<float>());
 }
 
 // Limit the troubled elements listed by stage `slot` (counter index).
@@ -489,6 +531,8 @@ void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, b
     a.flags = nullptr;
     a.list = g.list.as<int>();
     a.counters = g.counters.as<int>();
+    a.tin = g.traces_of(in[s]);
+    a.tout = g.traces_of(out[s]);
     if (timed)
       HIPCHECK(hipEventRecord(g.ev[2 * timed_index], g.stream));
     launch_stage(g, a);
@@ -564,6 +608,91 @@ double resident_speed(wasl_hip_engine &g) {
   double speed;
   std::memcpy(&speed, &h->s[0].lambda, sizeof(double));
   return speed;
+}
+
+// ------------------------------------------------------------- autotuner ----
+//
+// Tile shape of the FP64 Euler stage kernel (point split with 1, 2 or 4 lanes
+// per element, variable split with 4 or 8), keyed on dimension, equation,
+// order, precision and a mesh-size class. The first engine of a key times
+// every candidate on its own mesh -- residual-only launches of a free-stream
+// state, HIP events on its stream -- and the fastest serves every later engine
+// of that key in the process. WASL_HIP_AUTOTUNE=0 keeps choose_tile's table.
+
+struct TuneKey {
+  int dim, equation, order, precision, size_class;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dim, equation, order, precision, size_class) <
+           std::tie(o.dim, o.equation, o.order, o.precision, o.size_class);
+  }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+
+int size_class(int E) { return E < 8192 ? 0 : E < 131072 ? 1 : 2; }
+
+void launch_stage(const wasl_hip_engine &g, const whip::StageArgs &a);
+
+void autotune(wasl_hip_engine &g) {
+  if (const char *env = std::getenv("WASL_HIP_AUTOTUNE"))
+    if (std::atoi(env) == 0)
+      return;
+  const TuneKey key{g.dim, g.num.equation, g.p, g.num.precision, size_class(g.E)};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      g.stage_kernel = it->second.first;
+      g.lanes = it->second.second;
+      return;
+    }
+  }
+  // free stream (rho, u, v, p) = (1, 0.5, 0.25, 1): mode 0 only
+  std::vector<double> U(g.state_size(), 0.0);
+  const double gm = g.num.heat_capacity_ratio, s2 = std::sqrt(2.0);
+  const double cons[4] = {1.0, 0.5, 0.25, 1.0 / (gm - 1.0) + 0.5 * (0.25 + 0.0625)};
+  for (int v = 0; v < 4; ++v)
+    for (int e = 0; e < g.E; ++e)
+      U[static_cast<size_t>(v) * g.Np * g.E + e] = s2 * cons[v];
+  HIPCHECK(hipMemcpyAsync(g.w1.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
+                          g.stream));
+  whip::StageArgs a{};
+  a.uin = g.w1.as<double>();
+  a.un = g.w1.as<double>();
+  a.out = g.w2.as<double>();
+  a.cb = 1.0;
+  a.residual_only = 1;
+  hipEvent_t t0, t1;
+  HIPCHECK(hipEventCreate(&t0));
+  HIPCHECK(hipEventCreate(&t1));
+  const std::pair<int, int> candidates[] = {{1, 1}, {1, 2}, {1, 4}, {2, 4}, {2, 8}};
+  std::pair<int, int> best = {g.stage_kernel, g.lanes};
+  float best_ms = 0.0f;
+  for (const auto &c : candidates) {
+    g.stage_kernel = c.first;
+    g.lanes = c.second;
+    for (int i = 0; i < 2; ++i)
+      launch_stage(g, a);
+    HIPCHECK(hipEventRecord(t0, g.stream));
+    constexpr int kReps = 5;
+    for (int i = 0; i < kReps; ++i)
+      launch_stage(g, a);
+    HIPCHECK(hipEventRecord(t1, g.stream));
+    HIPCHECK(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+    if (best_ms == 0.0f || ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  g.stage_kernel = best.first;
+  g.lanes = best.second;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tuned.emplace(key, best);
 }
 
 void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hip_engine &g) {
@@ -746,6 +875,12 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   g.u.alloc(n, tot);
   g.w1.alloc(n, tot);
   g.w2.alloc(n, tot);
+  if (g.mixed()) {
+    const size_t tb = static_cast<size_t>(g.E) * 48 * sizeof(float);
+    g.tr_u.alloc(tb, tot);
+    g.tr_w1.alloc(tb, tot);
+    g.tr_w2.alloc(tb, tot);
+  }
   HIPCHECK(hipMemsetAsync(g.u.p, 0, n, g.stream));
   g.counters.alloc(4 * sizeof(int), tot);
   HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
@@ -755,6 +890,9 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
   g.result.alloc(sizeof(double), tot);
   HIPCHECK(hipStreamSynchronize(g.stream));
+  if (g.dim == 2 && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
+      num.lanes_per_element == 0)
+    autotune(g);
 }
 
 }  // namespace
@@ -1110,6 +1248,10 @@ int wasl_hip_residual(wasl_hip_engine *g, const double *state, double *rhs) {
     a.residual_only = 1;
     a.list = g->list.as<int>();
     a.counters = g->counters.as<int>();
+    if (g->mixed()) {
+      launch_traces(*g, a.uin, g->tr_w1.as<float>(), g->mxtable.as<float>(), g->stream);
+      a.tin = g->tr_w1.as<float>();
+    }
     launch_stage(*g, a);
     HIPCHECK(hipMemcpyAsync(rhs, g->w2.p, n, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
@@ -1157,6 +1299,9 @@ int wasl_hip_step(wasl_hip_engine *g, double dt) {
     ensure_events(*g, 3);
     int idx = 0;
     HIPCHECK(hipEventRecord(g->ev[0], g->stream));
+    if (g->mixed())
+      launch_traces(*g, g->u.as<double>(), g->tr_u.as<float>(), g->mxtable.as<float>(),
+                    g->stream);
     launch_step(*g, dt, nullptr, 0, false, idx);
     HIPCHECK(hipStreamSynchronize(g->stream));
     raise_device_error(*g);
@@ -1189,6 +1334,9 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     if (g->dim == 2)
       hipLaunchKernelGGL(whip::cfl_speed, dim3((g->E + 255) / 256), dim3(256), 0, g->stream,
                          g->u.as<double>(), g->E, g->Np, g->num.heat_capacity_ratio, dctl, 0);
+    if (g->mixed())
+      launch_traces(*g, g->u.as<double>(), g->tr_u.as<float>(), g->mxtable.as<float>(),
+                    g->stream);
     int64_t rows = 0;
     const auto record = [&](uint64_t st, double t, double dt) {
       if (history && rows < history_capacity) {

@@ -124,6 +124,10 @@ struct StageArgs {
   uint8_t *flags;             // [E] or null
   int *list;                  // troubled elements
   int *counters;              // [3] per-stage troubled counts
+  // mixed precision: FP32 edge traces [E][3 edges][4 points][4 variables] of
+  // uin (read for neighbours) and of out (written for the next stage, or null)
+  const float *tin;
+  float *tout;
 };
 
 // ------------------------------------------------------------- helpers ----
@@ -229,37 +233,46 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
                                       bool leader) {
   if (!ctl)
     return true;
-  if (ctl->err)
+  // Every field is read in one batch before any decision: one memory round
+  // trip instead of a chain of dependent ones.
+  StepSlot &slot = ctl->s[parity];
+  const int err = ctl->err;
+  const double time = slot.time, slot_dt = slot.dt;
+  const unsigned long long step = slot.step, lambda = slot.lambda;
+  const int slot_active = slot.active, bad = slot.bad;
+  const double end_time = ctl->end_time, cfl_length = ctl->cfl_length;
+  const double order_factor = ctl->order_factor;
+  const unsigned long long max_steps = ctl->max_steps;
+  const int fixed_speed = ctl->fixed_speed;
+  if (err)
     return false;
-  StepSlot &cur = ctl->s[parity];
   if (stage != 0) {
-    if (!cur.active)
+    if (!slot_active)
       return false;
-    dt = cur.dt;
+    dt = slot_dt;
     return true;
   }
-  const double time = cur.time;
-  const bool active = time < ctl->end_time && cur.step < ctl->max_steps;
+  const bool active = time < end_time && step < max_steps;
   if (!active) {
     if (leader && threadIdx.x == 0) {
-      cur.active = 0;
+      slot.active = 0;
       StepSlot &nxt = ctl->s[parity ^ 1];
       nxt.time = time;
-      nxt.step = cur.step;
-      nxt.lambda = cur.lambda;
-      nxt.bad = cur.bad;
+      nxt.step = step;
+      nxt.lambda = lambda;
+      nxt.bad = bad;
       nxt.active = 0;
     }
     return false;
   }
-  if (cur.bad) {
+  if (bad) {
     if (threadIdx.x == 0)
       raise_error(ctl, nullptr, kErrUnphysical);
     return false;
   }
-  const double speed = __longlong_as_double(static_cast<long long>(cur.lambda));
-  const double remaining = ctl->end_time - time;
-  const double stable = speed > 0.0 ? ctl->cfl_length / (speed * ctl->order_factor) : 0.0;
+  const double speed = __longlong_as_double(static_cast<long long>(lambda));
+  const double remaining = end_time - time;
+  const double stable = speed > 0.0 ? cfl_length / (speed * order_factor) : 0.0;
   dt = fmin(stable > 0.0 ? stable : remaining, remaining);
   if (dt <= 0.0 || time + dt == time) {
     if (threadIdx.x == 0)
@@ -267,11 +280,11 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
     return false;
   }
   if (leader && threadIdx.x == 0) {
-    cur.dt = dt;
-    cur.active = 1;
+    slot.dt = dt;
+    slot.active = 1;
     ctl->last_dt = dt;
     StepSlot &nxt = ctl->s[parity ^ 1];
-    nxt.lambda = ctl->fixed_speed ? cur.lambda : 0ull;
+    nxt.lambda = fixed_speed ? lambda : 0ull;
     nxt.bad = 0;
   }
   return true;
@@ -876,7 +889,9 @@ constexpr int kMetaReverse = 128;   // the two sides order the face points oppos
 template <int P> struct Mx {
   static constexpr int Np = Tri<P>::Np, NQ = Tri<P>::NQ, NE = Tri<P>::NE;
   static constexpr int MT = (NQ + 15) / 16;   // point tiles of 16
-  static constexpr int KS = (Np + 3) / 4;     // K steps of 4 modes
+  // K steps over modes: step r takes modes 4g + r from row group g, so a lane
+  // feeds its own modes 4g .. 4g + 3, the layout the epilogue holds
+  static constexpr int KS = Np < 4 ? Np : 4;
   // per-lane operand tables, floats, each [..][64]
   static constexpr int kAvol = 0;                         // [MT][KS]
   static constexpr int kAgrad = kAvol + MT * KS * 64;     // [MT][4 r][2 d]
@@ -892,6 +907,10 @@ template <int P> struct Mx {
 };
 
 typedef float mx_f4 __attribute__((ext_vector_type(4)));
+
+// Waves per SIMD the mixed stage kernels are built for (register budget 512 /
+// waves): every global read sits in registers from the start of the kernel.
+constexpr int kMxWaves = 2;
 typedef short mx_s4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ mx_f4 mfma4(float a, float b, mx_f4 c) {
@@ -988,26 +1007,33 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const int e = valid ? e_raw : E - 1;
   const float g = static_cast<float>(mesh.gamma), g1 = g - 1.0f;
 
-  // Own coefficients as B operands (element j, modes grp + 4s; bf16: 4 grp + i),
-  // geometry, the edge record of edge `grp` and the neighbour's coefficients,
-  // all requested ahead of the step-control read (every address in bounds).
-  float cb[4][KS];
-  mx_s4 cb16[4];
+  // Every global read is issued here, ahead of the step-control read, so the
+  // kernel pays about two memory round trips (the neighbour's trace waits for
+  // its index) instead of one per phase; all addresses are in bounds. The
+  // element's own modes 4 grp .. 4 grp + 3 of each variable (FP64, kept for
+  // the RK combination), U^n of the same outputs, its metrics, the record of
+  // edge `grp`, the neighbour's trace on that edge (64 contiguous bytes) and
+  // this thread's share of the operand table.
+  double c64[4][4], un64[4][4];
+  float cf[4][4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int m = 4 * s + grp;
-      cb[v][s] = m < Np ? static_cast<float>(a.uin[(v * Np + m) * E + e]) : 0.0f;
+    for (int r = 0; r < 4; ++r) {
+      const int m = 4 * grp + r;
+      const int idx = (v * Np + (m < Np ? m : 0)) * E + e;
+      c64[v][r] = m < Np ? a.uin[idx] : 0.0;
+      un64[v][r] = m < Np && a.ca != 0.0 ? a.un[idx] : 0.0;
+      cf[v][r] = static_cast<float>(c64[v][r]);
     }
-    if constexpr (BF16) {
+  constexpr int kPer = (kStaged + 255) / 256;
+  float tpre[kPer];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int m = 4 * grp + i;
-        cb16[v][i] = m < Np ? to_bf16(static_cast<float>(a.uin[(v * Np + m) * E + e])) : short(0);
-      }
-    }
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    tpre[i] = idx < kStaged ? mxtab[idx] : 0.0f;
   }
+  const double invdet = mesh.geo[4 * E + e];
   const float gA = static_cast<float>(mesh.geo[e]), gB = static_cast<float>(mesh.geo[E + e]),
               gC = static_cast<float>(mesh.geo[2 * E + e]),
               gD = static_cast<float>(mesh.geo[3 * E + e]);
@@ -1018,7 +1044,18 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const float ny = static_cast<float>(mesh.fn[(3 * k + 1) * E + e]);
   const float scale = static_cast<float>(mesh.fn[(3 * k + 2) * E + e]);
   const bool boundary = meta & kMetaBoundary;
-  const int src = boundary ? e : nbr;
+  const bool is_left = meta & kMetaLeft;
+  const bool rev = meta & kMetaReverse;
+  const int kl = meta_kl(meta);
+  const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
+  float4 ntr[4];
+  {
+    const float4 *src = reinterpret_cast<const float4 *>(
+        a.tin + (static_cast<size_t>(boundary ? e : nbr) * 3 + (boundary ? 0 : kn)) * 16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      ntr[s] = src[s];
+  }
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
@@ -1027,8 +1064,12 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   if (a.sensor && leader && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
-  for (int i = threadIdx.x; i < kStaged; i += 256)
-    tab[i] = mxtab[i];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    if (idx < kStaged)
+      tab[idx] = tpre[i];
+  }
   __syncthreads();
 
   mx_f4 R[4];
@@ -1039,6 +1080,13 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   // ---- volume: states at points (one product per variable, shared A),
   // pointwise fluxes once per point, flux contraction
   if constexpr (P > 0) {
+    mx_s4 cb16[4];
+    if constexpr (BF16) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        cb16[v] = mx_s4{to_bf16(cf[v][0]), to_bf16(cf[v][1]), to_bf16(cf[v][2]),
+                        to_bf16(cf[v][3])};
+    }
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
       mx_f4 S[4];
@@ -1056,7 +1104,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
           const float av = tab[X::kAvol + (t * KS + s) * 64 + lane];
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            S[v] = mfma4(av, cb[v][s], S[v]);
+            S[v] = mfma4(av, cf[v][s], S[v]);
         }
       }
       mx_f4 Fxi[4], Feta[4];
@@ -1100,14 +1148,9 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     }
   }
 
-  // ---- faces: own traces (rows 4k + s) on the matrix cores; neighbour
-  // traces, HLLC and weights on the vector ALU; lifting on the matrix cores.
-  float cn[4][Np];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
-      cn[v][m] = static_cast<float>(a.uin[(v * Np + m) * E + src]);
+  // ---- faces: own traces (rows 4k + s) on the matrix cores, bit-identical to
+  // the traces this element's neighbours read; HLLC once per face point and
+  // side on the vector ALU; lifting on the matrix cores.
   mx_f4 T[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -1117,28 +1160,16 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     const float af = tab[X::kAface + s * 64 + lane];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      T[v] = mfma4(af, cb[v][s], T[v]);
+      T[v] = mfma4(af, cf[v][s], T[v]);
   }
 
-  const bool is_left = meta & kMetaLeft;
-  const bool rev = meta & kMetaReverse;
-  const int kl = meta_kl(meta);
-  const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    F4 own = {T[0][s], T[1][s], T[2][s], T[3][s]};
+    const F4 own = {T[0][s], T[1][s], T[2][s], T[3][s]};
     F4 oth = own;   // transmissive: exterior = own trace
     if (!boundary) {
-      const float *b = tab + X::kBe + (kn * NE + (rev ? NE - 1 - s : s)) * Np;
-      float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int m = 0; m < Np; ++m) {
-        const float bm = b[m];
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          x[v] += cn[v][m] * bm;
-      }
-      oth = {x[0], x[1], x[2], x[3]};
+      const float4 n4 = ntr[rev ? NE - 1 - s : s];
+      oth = {n4.x, n4.y, n4.z, n4.w};
     } else if (mesh.bdonor) {
       const int slot = meta_slot(meta);
       const int d = mesh.bdonor[slot * NE + s];
@@ -1169,34 +1200,56 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   }
 
   // ---- finalise: lane holds R[v][m = 4 grp + r] of element j, FP64 from here
-  const double invdet = mesh.geo[4 * E + e];
-  double total = 0.0, high = 0.0, avg[4] = {0.0, 0.0, 0.0, 0.0};
+  double y[4][4];
+  double total = 0.0, high = 0.0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = 4 * grp + r;
+      y[v][r] = 0.0;
       if (m < Np) {
         const double res = static_cast<double>(R[v][r]) * invdet;
         const int idx = (v * Np + m) * E + e;
-        double y;
+        double x;
         if (a.residual_only) {
-          y = res;
+          x = res;
         } else {
-          y = a.cb * (a.uin[idx] + dt * res);
+          x = a.cb * (c64[v][r] + dt * res);
           if (a.ca != 0.0)
-            y += a.ca * a.un[idx];
+            x += a.ca * un64[v][r];
         }
         if (valid)
-          a.out[idx] = y;
+          a.out[idx] = x;
+        y[v][r] = x;
         if (v == 0) {
-          total += y * y;
+          total += x * x;
           if (m >= a.highest_begin)
-            high += y * y;
+            high += x * x;
         }
-        if (m == 0)
-          avg[v] = y;
       }
+    }
+  }
+
+  if (a.tout) {
+    // FP32 edge traces of the new state for the next stage's neighbours: the
+    // same product, operands and order as the own traces above.
+    mx_f4 Tn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      Tn[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const float af = tab[X::kAface + s * 64 + lane];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        Tn[v] = mfma4(af, static_cast<float>(y[v][s]), Tn[v]);
+    }
+    if (valid && grp < 3) {
+      float4 *dst = reinterpret_cast<float4 *>(a.tout + (static_cast<size_t>(e) * 3 + grp) * 16);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        dst[s] = make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]);
     }
   }
 
@@ -1221,7 +1274,8 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     double speed = 0.0;
     if (grp == 0 && valid) {
       const double is2 = 1.0 / sqrt(2.0);
-      const double r = is2 * avg[0], mx = is2 * avg[1], my = is2 * avg[2], en = is2 * avg[3];
+      const double r = is2 * y[0][0], mx = is2 * y[1][0], my = is2 * y[2][0],
+                   en = is2 * y[3][0];
       const double inv = 1.0 / r;
       const double u = mx * inv, w = my * inv;
       const double p = (mesh.gamma - 1.0) * (en - 0.5 * (mx * u + my * w));
@@ -1245,8 +1299,48 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   stage_epilogue(ctl, a.parity, a.stage, dt, leader);
 }
 
+// FP32 edge traces [E][3][4][4] of a state, by the product the stage epilogue
+// uses (bit-identical to it): at the start of a run and for a state that
+// arrives from the host.
+template <int P>
+__device__ __forceinline__ void mx_traces_body(const double *U, int E, float *T,
+                                               const float *mxtab, int block) {
+  using X = Mx<P>;
+  constexpr int Np = X::Np, KS = X::KS;
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 15, grp = lane >> 4;
+  const int e_raw = block * 64 + (threadIdx.x >> 6) * 16 + j;
+  const int e = e_raw < E ? e_raw : E - 1;
+  mx_f4 Tn[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    Tn[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const float af = mxtab[X::kAface + s * 64 + lane];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = 4 * grp + s;
+      const float c = m < Np ? static_cast<float>(U[(v * Np + m) * E + e]) : 0.0f;
+      Tn[v] = mfma4(af, c, Tn[v]);
+    }
+  }
+  if (e_raw < E && grp < 3) {
+    float4 *dst = reinterpret_cast<float4 *>(T + (static_cast<size_t>(e) * 3 + grp) * 16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      dst[s] = make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]);
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void mx_traces(const double *U, int E, float *T,
+                                                 const float *mxtab) {
+  mx_traces_body<P>(U, E, T, mxtab, blockIdx.x);
+}
+
 template <int P, bool BF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMxWaves)))
 void euler_stage_mx(EulerMesh mesh, StageArgs a, const float *mxtab) {
   __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
   __shared__ double red[4];
@@ -1264,6 +1358,7 @@ void euler_stage_mx(EulerMesh mesh, StageArgs a, const float *mxtab) {
 struct BatchCase {
   EulerMesh mesh;              // 2D Euler geometry and edge records
   double *u, *w1, *w2;         // state and the two stage buffers
+  float *tu, *tw1, *tw2;       // their FP32 edge traces (mixed precision)
   RunCtl *ctl;
   int *list, *counters;        // troubled elements (limiting)
   int sensor;
@@ -1293,11 +1388,13 @@ __device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b
   a.highest_begin = c.highest_begin;
   a.list = c.list;
   a.counters = c.counters;
+  a.tin = b.stage == 0 ? c.tu : b.stage == 1 ? c.tw1 : c.tw2;
+  a.tout = b.stage == 0 ? c.tw1 : b.stage == 1 ? c.tw2 : c.tu;
   return a;
 }
 
 template <int P, bool BF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMxWaves)))
 void euler_stage_mx_batch(const BatchCase *cases, const int2 *tiles, BatchStage b,
                           const float *mxtab) {
   __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
@@ -1803,9 +1900,12 @@ __device__ __forceinline__ void limit_body(const LimiterDev L, const double *U, 
       dst[v * Np + m] = m == 0 ? o[v][m] : o[v][m] * ps;
 }
 
+// Writes the limited coefficients back; with a trace buffer (mixed
+// precision) also the FP32 edge traces of the limited element.
 template <int P>
 __device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
-                                             const int *count, RunCtl *ctl, int parity, int idx) {
+                                             const int *count, RunCtl *ctl, int parity, int idx,
+                                             float *tr = nullptr, const float *mxtab = nullptr) {
   constexpr int Np = Tri<P>::Np;
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
@@ -1818,6 +1918,24 @@ __device__ __forceinline__ void scatter_body(double *U, int E, const double *sid
 #pragma unroll
     for (int m = 0; m < Np; ++m)
       U[(v * Np + m) * E + e] = src[v * Np + m];
+  if (tr) {
+    using X = Mx<P>;
+    constexpr int NE = X::NE;
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < 4; ++q) {
+        float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (q < NE) {
+          const float *b = mxtab + X::kBe + (k * NE + q) * Np;
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              x[v] += static_cast<float>(src[v * Np + m]) * b[m];
+        }
+        reinterpret_cast<float4 *>(tr + (static_cast<size_t>(e) * 3 + k) * 16)[q] =
+            make_float4(x[0], x[1], x[2], x[3]);
+      }
+  }
 }
 
 template <int P>
@@ -1830,8 +1948,10 @@ __global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double 
 template <int P>
 __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
                                                        const int *list, const int *count,
-                                                       RunCtl *ctl, int parity) {
-  scatter_body<P>(U, E, side, list, count, ctl, parity, blockIdx.x * blockDim.x + threadIdx.x);
+                                                       RunCtl *ctl, int parity, float *tr,
+                                                       const float *mxtab) {
+  scatter_body<P>(U, E, side, list, count, ctl, parity, blockIdx.x * blockDim.x + threadIdx.x,
+                  tr, mxtab);
 }
 
 // Batched limiter: tiles of 64 troubled-list slots per (case, chunk); the
@@ -1839,6 +1959,7 @@ __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const d
 struct BatchLimit {
   LimiterDev L;
   double *u, *w1, *w2;
+  float *tu, *tw1, *tw2;       // edge traces of u, w1, w2 (mixed precision)
   double *side;
   int *list, *counters;
   RunCtl *ctl;
@@ -1847,6 +1968,10 @@ struct BatchLimit {
 
 __device__ __forceinline__ double *batch_out(const BatchLimit &c, int stage) {
   return stage == 0 ? c.w1 : stage == 1 ? c.w2 : c.u;
+}
+
+__device__ __forceinline__ float *batch_out_traces(const BatchLimit &c, int stage) {
+  return stage == 0 ? c.tw1 : stage == 1 ? c.tw2 : c.tu;
 }
 
 template <int P>
@@ -1860,11 +1985,12 @@ __global__ __launch_bounds__(64) void limit_troubled_batch(const BatchLimit *cas
 
 template <int P>
 __global__ __launch_bounds__(64) void scatter_limited_batch(const BatchLimit *cases,
-                                                            const int2 *tiles, BatchStage b) {
+                                                            const int2 *tiles, BatchStage b,
+                                                            const float *mxtab) {
   const int2 t = tiles[blockIdx.x];
   const BatchLimit &c = cases[t.x];
   scatter_body<P>(batch_out(c, b.stage), c.L.E, c.side, c.list, c.counters + b.stage, c.ctl,
-                  b.parity, t.y * 64 + threadIdx.x);
+                  b.parity, t.y * 64 + threadIdx.x, batch_out_traces(c, b.stage), mxtab);
 }
 
 }  // namespace whip

@@ -188,6 +188,8 @@ void admit_ctl(wasl_hip_server &s, Request &r) {
     hipLaunchKernelGGL(whip::cfl_speed, dim3((g.E + 255) / 256), dim3(256), 0, s.stream,
                        g.u.as<double>(), g.E, g.Np, g.num.heat_capacity_ratio,
                        g.ctl.as<whip::RunCtl>(), 0);
+  if (g.mixed())
+    launch_traces(g, g.u.as<double>(), g.tr_u.as<float>(), s.mxtab[g.p].as<float>(), s.stream);
   HIPCHECK(hipGetLastError());
   // the copy above reads the host struct: complete it before it goes out of scope
   HIPCHECK(hipStreamSynchronize(s.stream));
@@ -228,6 +230,9 @@ void rebuild(wasl_hip_server &s) {
         c.u = g.u.as<double>();
         c.w1 = g.w1.as<double>();
         c.w2 = g.w2.as<double>();
+        c.tu = g.tr_u.as<float>();
+        c.tw1 = g.tr_w1.as<float>();
+        c.tw2 = g.tr_w2.as<float>();
         c.ctl = g.ctl.as<whip::RunCtl>();
         const bool lim = limiting(g);
         c.sensor = lim ? 1 : 0;
@@ -254,6 +259,9 @@ void rebuild(wasl_hip_server &s) {
           m.u = g.u.as<double>();
           m.w1 = g.w1.as<double>();
           m.w2 = g.w2.as<double>();
+          m.tu = g.tr_u.as<float>();
+          m.tw1 = g.tr_w1.as<float>();
+          m.tw2 = g.tr_w2.as<float>();
           m.side = g.side.as<double>();
           m.list = g.list.as<int>();
           m.counters = g.counters.as<int>();
@@ -340,7 +348,7 @@ template <int P> void launch_limit_class(wasl_hip_server &s, const wasl_hip_serv
   hipLaunchKernelGGL((whip::limit_troubled_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream, cases,
                      tiles, b);
   hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream,
-                     cases, tiles, b);
+                     cases, tiles, b, s.mxtab[P].as<float>());
 }
 
 template <typename F> void by_order(int p, F &&f) {
