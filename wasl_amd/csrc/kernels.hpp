@@ -621,7 +621,7 @@ __device__ __forceinline__ void lane_exchange_fence() {
 // the scalar cache; edge tables depend on each element's face configuration
 // and are staged in LDS.
 template <int P, int H>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P >= 3 ? 3 : 4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   using D = Tri<P>;
   constexpr int Np = D::Np, NE = D::NE;
@@ -641,7 +641,11 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
 
-  // Loads ahead of the step-control read (all in bounds).
+  // Every global read of the stage is issued here, ahead of the step-control
+  // read (all addresses in bounds): own coefficients, metrics, the records
+  // and neighbour coefficients of this quad's edges, U^n, 1/detJ and this
+  // thread's share of the edge tables. The kernel then pays about two memory
+  // round trips (neighbour data waits for its index), not one per phase.
   double c[Np], acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -650,6 +654,40 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   }
   const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
                gD = mesh.geo[3 * E + e];
+  constexpr int KE = (3 + H - 1) / H;   // edges per quad
+  // Edges whose neighbour coefficients are fetched here; with one quad per
+  // element at P3 the rest are fetched one edge ahead inside the face loop,
+  // which keeps the kernel within its register budget.
+  constexpr int KP = (H == 1 && Np > 6) ? 1 : KE;
+  int emeta[KE], esrc[KE];
+  double enx[KE], eny[KE], escale[KE], env[KP][Np];
+#pragma unroll
+  for (int i = 0; i < KE; ++i) {
+    const int k = h + i * H < 3 ? h + i * H : 0;
+    emeta[i] = mesh.meta[k * E + e];
+    const int nb = mesh.nbr[k * E + e];
+    enx[i] = mesh.fn[(3 * k) * E + e];
+    eny[i] = mesh.fn[(3 * k + 1) * E + e];
+    escale[i] = mesh.fn[(3 * k + 2) * E + e];
+    esrc[i] = (emeta[i] & kMetaBoundary) ? e : nb;
+    if (i < KP) {
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        env[i][m] = a.uin[(v * Np + m) * E + esrc[i]];
+    }
+  }
+  double un[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m)
+    un[m] = a.ca != 0.0 ? a.un[(v * Np + m) * E + e] : 0.0;
+  const double invdet = mesh.geo[4 * E + e];
+  constexpr int kPer = (D::kEdgeTable + 255) / 256;
+  double tpre[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    tpre[i] = idx < D::kEdgeTable ? mesh.table[D::kEdge + idx] : 0.0;
+  }
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
@@ -658,8 +696,12 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
-  for (int i = threadIdx.x; i < D::kEdgeTable; i += 256)
-    tab[i] = mesh.table[D::kEdge + i];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    if (idx < D::kEdgeTable)
+      tab[idx] = tpre[i];
+  }
   __syncthreads();
 
   static_assert(D::kVTable <= kVTableMax, "volume table slot too small");
@@ -713,13 +755,11 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
 
   // Faces: quad h takes edges h, h + H, ...; lane v builds both traces of
   // variable v at the edge points, lane q < NE evaluates HLLC at point q.
-#pragma unroll 1
-  for (int k = h; k < 3; k += H) {
-    const int meta = mesh.meta[k * E + e];
-    const int nb = mesh.nbr[k * E + e];
-    const double nx = mesh.fn[(3 * k) * E + e];
-    const double ny = mesh.fn[(3 * k + 1) * E + e];
-    const double scale = mesh.fn[(3 * k + 2) * E + e];
+  // nv holds the neighbour coefficients of the edge; when next >= 0 the
+  // coefficients of element next are fetched into it once the traces are
+  // built, overlapping the flux and lifting work of this edge.
+  auto face = [&](const int meta, const double nx, const double ny, const double scale,
+                  double *nv, const int next) {
     const bool is_left = meta & kMetaLeft;
     const bool boundary = meta & kMetaBoundary;
     const int kl = meta_kl(meta), kr = meta_kr(meta), fl = meta_flip(meta);
@@ -727,12 +767,6 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
     const double *br = tab + (D::kBr - D::kEdge) + ((kl * 3 + kr) * 2 + fl) * NE * Np;
     const double *own = is_left ? bl : br;
     const double *oth = is_left ? br : bl;
-    double nv[Np];
-    if (!boundary) {
-#pragma unroll
-      for (int m = 0; m < Np; ++m)
-        nv[m] = a.uin[(v * Np + m) * E + nb];
-    }
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
       double si = 0.0;
@@ -757,6 +791,11 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
       xq[q][v] = si;
       xo[q][v] = so;
     }
+    if (next >= 0) {
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        nv[m] = a.uin[(v * Np + m) * E + next];
+    }
     lane_exchange_fence();
     if (v < NE) {
       const C4 si = {xq[v][0], xq[v][1], xq[v][2], xq[v][3]};
@@ -780,6 +819,22 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
         acc[m] += val * own[q * Np + m];
     }
     lane_exchange_fence();
+  };
+  if constexpr (KP == KE) {
+#pragma unroll
+    for (int i = 0; i < KE; ++i)
+      if (h + i * H < 3)
+        face(emeta[i], enx[i], eny[i], escale[i], env[i], -1);
+  } else {
+    static_assert(KP == 1 && KE == 3, "pipelined faces assume one quad per element");
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const int meta = i == 0 ? emeta[0] : i == 1 ? emeta[1] : emeta[2];
+      const double nx = i == 0 ? enx[0] : i == 1 ? enx[1] : enx[2];
+      const double ny = i == 0 ? eny[0] : i == 1 ? eny[1] : eny[2];
+      const double sc = i == 0 ? escale[0] : i == 1 ? escale[1] : escale[2];
+      face(meta, nx, ny, sc, env[0], i == 0 ? esrc[1] : i == 1 ? esrc[2] : -1);
+    }
   }
 
   if constexpr (H == 2) {
@@ -789,7 +844,6 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
   }
 
   // Finalise variable v (quad 0 of the element stores).
-  const double invdet = mesh.geo[4 * E + e];
   double y[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -800,7 +854,7 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
     } else {
       y[m] = a.cb * (c[m] + dt * r);
       if (a.ca != 0.0)
-        y[m] += a.ca * a.un[idx];
+        y[m] += a.ca * un[m];
     }
     if (valid && h == 0)
       a.out[idx] = y[m];
@@ -910,7 +964,7 @@ typedef float mx_f4 __attribute__((ext_vector_type(4)));
 
 // Waves per SIMD the mixed stage kernels are built for (register budget 512 /
 // waves): every global read sits in registers from the start of the kernel.
-constexpr int kMxWaves = 2;
+constexpr int kMxWaves = 3;
 typedef short mx_s4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ mx_f4 mfma4(float a, float b, mx_f4 c) {
@@ -1023,7 +1077,6 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const int m = 4 * grp + r;
       const int idx = (v * Np + (m < Np ? m : 0)) * E + e;
       c64[v][r] = m < Np ? a.uin[idx] : 0.0;
-      un64[v][r] = m < Np && a.ca != 0.0 ? a.un[idx] : 0.0;
       cf[v][r] = static_cast<float>(c64[v][r]);
     }
   constexpr int kPer = (kStaged + 255) / 256;
@@ -1115,12 +1168,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
         const float inv = rcpf(real ? rho : 1.0f);
         const float u = mx * inv, w = my * inv;
         const float p = g1 * (en - 0.5f * (mx * u + my * w));
-        const float fx[4] = {mx, mx * u + p, my * u, (en + p) * u};
-        const float fy[4] = {my, mx * w, my * w + p, (en + p) * w};
+        // reference-frame fluxes through the contravariant velocities
+        // U_xi = A u + B v, U_eta = C u + D v
+        const float uxi = gA * u + gB * w, ueta = gC * u + gD * w, enp = en + p;
+        const float fxi[4] = {rho * uxi, mx * uxi + gA * p, my * uxi + gB * p, enp * uxi};
+        const float feta[4] = {rho * ueta, mx * ueta + gC * p, my * ueta + gD * p, enp * ueta};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          Fxi[v][r] = real ? gA * fx[v] + gB * fy[v] : 0.0f;
-          Feta[v][r] = real ? gC * fx[v] + gD * fy[v] : 0.0f;
+          Fxi[v][r] = real ? fxi[v] : 0.0f;
+          Feta[v][r] = real ? feta[v] : 0.0f;
         }
       }
       if constexpr (BF16) {
@@ -1147,6 +1203,16 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       }
     }
   }
+
+  // U^n of the owned outputs (stages 2 and 3): requested here, while the
+  // faces are evaluated, rather than held through the volume phase.
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 4 * grp + r;
+      un64[v][r] = m < Np && a.ca != 0.0 ? a.un[(v * Np + m) * E + e] : 0.0;
+    }
 
   // ---- faces: own traces (rows 4k + s) on the matrix cores, bit-identical to
   // the traces this element's neighbours read; HLLC once per face point and
@@ -1200,14 +1266,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   }
 
   // ---- finalise: lane holds R[v][m = 4 grp + r] of element j, FP64 from here
-  double y[4][4];
+  float yf[4][4];
+  double avg[4] = {0.0, 0.0, 0.0, 0.0};
   double total = 0.0, high = 0.0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = 4 * grp + r;
-      y[v][r] = 0.0;
+      yf[v][r] = 0.0f;
       if (m < Np) {
         const double res = static_cast<double>(R[v][r]) * invdet;
         const int idx = (v * Np + m) * E + e;
@@ -1221,7 +1288,9 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
         }
         if (valid)
           a.out[idx] = x;
-        y[v][r] = x;
+        yf[v][r] = static_cast<float>(x);
+        if (r == 0)
+          avg[v] = x;
         if (v == 0) {
           total += x * x;
           if (m >= a.highest_begin)
@@ -1243,7 +1312,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const float af = tab[X::kAface + s * 64 + lane];
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        Tn[v] = mfma4(af, static_cast<float>(y[v][s]), Tn[v]);
+        Tn[v] = mfma4(af, yf[v][s], Tn[v]);
     }
     if (valid && grp < 3) {
       float4 *dst = reinterpret_cast<float4 *>(a.tout + (static_cast<size_t>(e) * 3 + grp) * 16);
@@ -1274,8 +1343,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     double speed = 0.0;
     if (grp == 0 && valid) {
       const double is2 = 1.0 / sqrt(2.0);
-      const double r = is2 * y[0][0], mx = is2 * y[1][0], my = is2 * y[2][0],
-                   en = is2 * y[3][0];
+      const double r = is2 * avg[0], mx = is2 * avg[1], my = is2 * avg[2], en = is2 * avg[3];
       const double inv = 1.0 / r;
       const double u = mx * inv, w = my * inv;
       const double p = (mesh.gamma - 1.0) * (en - 0.5 * (mx * u + my * w));
@@ -1696,208 +1764,190 @@ __device__ __forceinline__ double pressure_of(double g, double r, double mx, dou
   return (g - 1.0) * (E - 0.5 * (mx * u + my * v));
 }
 
+// w[v] for a run-time v without an indexed (scratch) access.
+__device__ __forceinline__ double pick4(const double (&w)[4], int v) {
+  return v == 0 ? w[0] : v == 1 ? w[1] : v == 2 ? w[2] : w[3];
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+  for (int o = 32; o > 0; o >>= 1)
+    x = fmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
 // Troubled elements from the list: characteristic simple WENO against the
 // transformed neighbours (all read from the unlimited state), then the
 // Zhang-Shu positivity limiter in FP64 (euler_modal_limiter.cpp:81-97,
 // euler_characteristic_weno.hpp:17-67, simple_weno.hpp:21-48,
 // euler_positivity.hpp:63-102). Results go to side[idx] for scatter().
+//
+// One wave per troubled element: lane (v, m) owns coefficient m of variable
+// v, lane q < NC one positivity check point. Every sum keeps the reference's
+// sequential order (one lane walks it through LDS) and minima are exact; the
+// modes, the candidates and the 64-step bisections of the check points
+// proceed side by side. Blocks of 64 threads take list entries first,
+// first + stride, ...
 template <int P>
-__device__ __forceinline__ void limit_body(const LimiterDev L, const double *U, double *side,
-                                           const int *list, const int *count, RunCtl *ctl,
-                                           int parity, int *err, int idx) {
-  constexpr int Np = Tri<P>::Np, NC = Tri<P>::NC;
+__device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double *U, double *side,
+                                                const int *list, const int *count, RunCtl *ctl,
+                                                int parity, int *err, int first, int stride) {
+  constexpr int Np = Tri<P>::Np, NC = Tri<P>::NC, NV = 4 * Np;
+  static_assert(NV <= 64 && NC <= 64, "one wave per element");
+  __shared__ double t[4][Np], tc[4][Np], cv[4][Np], cc[3][4][Np], lim[4][Np], o[4][Np];
+  __shared__ double w0[4], wk[3][4], wsum[4];
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
-  if (idx >= *count)
-    return;
-  const int e = list[idx];
+  const int n = *count;
+  const int lane = threadIdx.x;
+  const bool vm = lane < NV;
+  const int v = vm ? lane / Np : 0, m = vm ? lane % Np : 0;
   const int E = L.E;
   const double g = L.gamma;
-  double t[4][Np];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
+  for (int idx = first; idx < n; idx += stride) {
+    const int e = list[idx];
+    if (vm)
       t[v][m] = U[(v * Np + m) * E + e];
-  const double s2 = sqrt(2.0);
-  const double avg[4] = {t[0][0] / s2, t[1][0] / s2, t[2][0] / s2, t[3][0] / s2};
-  const CharFrame f = char_frame(g, avg, L.nrm[2 * e], L.nrm[2 * e + 1]);
-
-  double tc[4][Np];
-#pragma unroll
-  for (int m = 0; m < Np; ++m) {
-    double w[4];
-    to_char(f, t[0][m], t[1][m], t[2][m], t[3][m], w);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      tc[k][m] = w[k];
-  }
-  int nbs[3];
-  int K = 0;
-  for (int k = 0; k < 3; ++k) {
-    const int nb = L.nbr[3 * e + k];
-    if (nb >= 0)
-      nbs[K++] = k;
-  }
-  double lim[4][Np];
-  if (K == 0) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w)
-#pragma unroll
-      for (int m = 0; m < Np; ++m)
-        lim[w][m] = tc[w][m];
-  } else {
-    constexpr double eps = 1e-12;
-    double w0[4], wk[3][4], wsum[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      double b = 0.0;
-#pragma unroll
-      for (int m = 1; m < Np; ++m)
-        b += tc[w][m] * tc[w][m];
-      const double d = eps + b;
-      w0[w] = 0.5 / (d * d);
-      wsum[w] = w0[w];
+    __syncthreads();
+    const double s2 = sqrt(2.0);
+    const double avg[4] = {t[0][0] / s2, t[1][0] / s2, t[2][0] / s2, t[3][0] / s2};
+    const CharFrame f = char_frame(g, avg, L.nrm[2 * e], L.nrm[2 * e + 1]);
+    if (vm) {
+      double w[4];
+      to_char(f, t[0][m], t[1][m], t[2][m], t[3][m], w);
+      tc[v][m] = pick4(w, v);
     }
-    const double ideal = 0.5 / static_cast<double>(K);
-    // Candidate j in characteristic variables: T_j U_nb with the element's own mean.
-    const auto candidate = [&](int j, double cc[4][Np]) {
-      const int k = nbs[j];
-      const int nb = L.nbr[3 * e + k];
-      const double *M = L.T + (static_cast<size_t>(e) * 3 + k) * Np * Np;
-      double cv[4][Np];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double src[Np];
-#pragma unroll
-        for (int sm = 0; sm < Np; ++sm)
-          src[sm] = U[(v * Np + sm) * E + nb];
-#pragma unroll
-        for (int tm = 0; tm < Np; ++tm) {
+    int K = 0;
+    for (int k = 0; k < 3; ++k)
+      if (L.nbr[3 * e + k] >= 0)
+        ++K;
+    __syncthreads();
+    if (K == 0) {
+      if (vm)
+        lim[v][m] = tc[v][m];
+    } else {
+      // candidate j in characteristic variables: T_j U_nb with the own mean
+      int j = 0;
+      for (int k = 0; k < 3; ++k) {
+        const int nb = L.nbr[3 * e + k];
+        if (nb < 0)
+          continue;
+        const double *M = L.T + (static_cast<size_t>(e) * 3 + k) * Np * Np;
+        if (vm) {
           double x = 0.0;
 #pragma unroll
           for (int sm = 0; sm < Np; ++sm)
-            x += M[tm * Np + sm] * src[sm];
-          cv[v][tm] = x;
+            x += M[m * Np + sm] * U[(v * Np + sm) * E + nb];
+          cv[v][m] = m == 0 ? t[v][0] : x;
         }
-        cv[v][0] = t[v][0];
+        __syncthreads();
+        if (vm) {
+          double w[4];
+          to_char(f, cv[0][m], cv[1][m], cv[2][m], cv[3][m], w);
+          cc[j][v][m] = pick4(w, v);
+        }
+        __syncthreads();
+        ++j;
       }
-#pragma unroll
-      for (int m = 0; m < Np; ++m) {
-        double w[4];
-        to_char(f, cv[0][m], cv[1][m], cv[2][m], cv[3][m], w);
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2)
-          cc[k2][m] = w[k2];
-      }
-    };
-    double cc[4][Np];
-    for (int j = 0; j < K; ++j) {
-      candidate(j, cc);
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
+      if (lane < 4) {
+        constexpr double eps = 1e-12;
+        const int w = lane;
         double b = 0.0;
-#pragma unroll
-        for (int m = 1; m < Np; ++m)
-          b += cc[w][m] * cc[w][m];
-        const double d = eps + b;
-        wk[j][w] = ideal / (d * d);
-        wsum[w] += wk[j][w];
+        for (int mm = 1; mm < Np; ++mm)
+          b += tc[w][mm] * tc[w][mm];
+        double d = eps + b;
+        w0[w] = 0.5 / (d * d);
+        double sum = w0[w];
+        const double ideal = 0.5 / static_cast<double>(K);
+        for (int j = 0; j < K; ++j) {
+          b = 0.0;
+          for (int mm = 1; mm < Np; ++mm)
+            b += cc[j][w][mm] * cc[j][w][mm];
+          d = eps + b;
+          wk[j][w] = ideal / (d * d);
+          sum += wk[j][w];
+        }
+        wsum[w] = sum;
+      }
+      __syncthreads();
+      if (vm) {
+        if (m == 0) {
+          lim[v][0] = tc[v][0];
+        } else {
+          double x = w0[v] * tc[v][m];
+          for (int j = 0; j < K; ++j)
+            x += wk[j][v] * cc[j][v][m];
+          lim[v][m] = x / wsum[v];
+        }
       }
     }
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      lim[w][0] = tc[w][0];
-#pragma unroll
-      for (int m = 1; m < Np; ++m)
-        lim[w][m] = w0[w] * tc[w][m];
+    __syncthreads();
+    // back to conserved variables, restoring every mean
+    if (vm) {
+      const double w[4] = {lim[0][m], lim[1][m], lim[2][m], lim[3][m]};
+      double out[4];
+      from_char(f, w, out);
+      o[v][m] = m == 0 ? t[v][0] : pick4(out, v);
     }
-    for (int j = 0; j < K; ++j) {
-      candidate(j, cc);
-#pragma unroll
-      for (int w = 0; w < 4; ++w)
-#pragma unroll
-        for (int m = 1; m < Np; ++m)
-          lim[w][m] += wk[j][w] * cc[w][m];
-    }
-#pragma unroll
-    for (int w = 0; w < 4; ++w)
-#pragma unroll
-      for (int m = 1; m < Np; ++m)
-        lim[w][m] = lim[w][m] / wsum[w];
-  }
-  // Back to conserved variables, restoring every mean.
-  double o[4][Np];
-#pragma unroll
-  for (int m = 0; m < Np; ++m) {
-    const double w[4] = {lim[0][m], lim[1][m], lim[2][m], lim[3][m]};
-    double out[4];
-    from_char(f, w, out);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      o[v][m] = out[v];
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-    o[v][0] = t[v][0];
-
-  // Positivity: density, then pressure by 64-step bisection, both scaling the
-  // high modes toward the mean.
-  const double is2 = 1.0 / sqrt(2.0);
-  const double a[4] = {o[0][0] * is2, o[1][0] * is2, o[2][0] * is2, o[3][0] * is2};
-  if (a[0] <= L.density_floor || pressure_of(g, a[0], a[1], a[2], a[3]) <= L.pressure_floor) {
-    raise_error(ctl, err, kErrPositivity);
-    return;
-  }
-  double ds = 1.0;
-  for (int q = 0; q < NC; ++q) {
-    const double *b = L.check + q * Np;
-    double r = 0.0;
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
-      r += o[0][m] * b[m];
-    if (r < L.density_floor)
-      ds = fmin(ds, (a[0] - L.density_floor) / (a[0] - r));
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int m = 1; m < Np; ++m)
-      o[v][m] *= ds;
-  double ps = 1.0;
-  for (int q = 0; q < NC; ++q) {
-    const double *b = L.check + q * Np;
-    double s[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double x = 0.0;
-#pragma unroll
-      for (int m = 0; m < Np; ++m)
-        x += o[v][m] * b[m];
-      s[v] = x;
-    }
-    if (pressure_of(g, s[0], s[1], s[2], s[3]) >= L.pressure_floor)
+    __syncthreads();
+    // positivity: density, then pressure by 64-step bisection
+    const double is2 = 1.0 / sqrt(2.0);
+    const double a[4] = {o[0][0] * is2, o[1][0] * is2, o[2][0] * is2, o[3][0] * is2};
+    if (a[0] <= L.density_floor || pressure_of(g, a[0], a[1], a[2], a[3]) <= L.pressure_floor) {
+      if (lane == 0)
+        raise_error(ctl, err, kErrPositivity);
+      __syncthreads();
       continue;
-    double lo = 0.0, hi = 1.0;
-    for (int it = 0; it < 64; ++it) {
-      const double mid = 0.5 * (lo + hi);
-      const double r = a[0] + mid * (s[0] - a[0]);
-      const double mx = a[1] + mid * (s[1] - a[1]);
-      const double my = a[2] + mid * (s[2] - a[2]);
-      const double en = a[3] + mid * (s[3] - a[3]);
-      if (pressure_of(g, r, mx, my, en) >= L.pressure_floor)
-        lo = mid;
-      else
-        hi = mid;
     }
-    ps = fmin(ps, lo);
-  }
-  double *dst = side + static_cast<size_t>(idx) * 4 * Np;
+    double ds = 1.0;
+    if (lane < NC) {
+      const double *b = L.check + lane * Np;
+      double r = 0.0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
+      for (int mm = 0; mm < Np; ++mm)
+        r += o[0][mm] * b[mm];
+      if (r < L.density_floor)
+        ds = (a[0] - L.density_floor) / (a[0] - r);
+    }
+    ds = wave_min(ds);
+    __syncthreads();
+    if (vm && m > 0)
+      o[v][m] *= ds;
+    __syncthreads();
+    double lo = 1.0;
+    if (lane < NC) {
+      const double *b = L.check + lane * Np;
+      double sv[4];
 #pragma unroll
-    for (int m = 0; m < Np; ++m)
+      for (int vv = 0; vv < 4; ++vv) {
+        double x = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < Np; ++mm)
+          x += o[vv][mm] * b[mm];
+        sv[vv] = x;
+      }
+      if (!(pressure_of(g, sv[0], sv[1], sv[2], sv[3]) >= L.pressure_floor)) {
+        double l = 0.0, h = 1.0;
+        for (int it = 0; it < 64; ++it) {
+          const double mid = 0.5 * (l + h);
+          const double r = a[0] + mid * (sv[0] - a[0]);
+          const double mx = a[1] + mid * (sv[1] - a[1]);
+          const double my = a[2] + mid * (sv[2] - a[2]);
+          const double en = a[3] + mid * (sv[3] - a[3]);
+          if (pressure_of(g, r, mx, my, en) >= L.pressure_floor)
+            l = mid;
+          else
+            h = mid;
+        }
+        lo = l;
+      }
+    }
+    const double ps = wave_min(lo);
+    if (vm) {
+      double *dst = side + static_cast<size_t>(idx) * 4 * Np;
       dst[v * Np + m] = m == 0 ? o[v][m] : o[v][m] * ps;
+    }
+    __syncthreads();
+  }
 }
 
 // Writes the limited coefficients back; with a trace buffer (mixed
@@ -1942,7 +1992,7 @@ template <int P>
 __global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double *U, double *side,
                                                      const int *list, const int *count,
                                                      RunCtl *ctl, int parity, int *err) {
-  limit_body<P>(L, U, side, list, count, ctl, parity, err, blockIdx.x * blockDim.x + threadIdx.x);
+  limit_wave_body<P>(L, U, side, list, count, ctl, parity, err, blockIdx.x, gridDim.x);
 }
 
 template <int P>
@@ -1974,13 +2024,17 @@ __device__ __forceinline__ float *batch_out_traces(const BatchLimit &c, int stag
   return stage == 0 ? c.tw1 : stage == 1 ? c.tw2 : c.tu;
 }
 
+// Wave-per-element limiter blocks of one case in a batch (tiles (case, b),
+// b < limit_blocks(E)).
+__host__ __device__ inline int limit_blocks(int E) { return E < 1024 ? E : 1024; }
+
 template <int P>
 __global__ __launch_bounds__(64) void limit_troubled_batch(const BatchLimit *cases,
                                                            const int2 *tiles, BatchStage b) {
   const int2 t = tiles[blockIdx.x];
   const BatchLimit &c = cases[t.x];
-  limit_body<P>(c.L, batch_out(c, b.stage), c.side, c.list, c.counters + b.stage, c.ctl, b.parity,
-                c.err, t.y * 64 + threadIdx.x);
+  limit_wave_body<P>(c.L, batch_out(c, b.stage), c.side, c.list, c.counters + b.stage, c.ctl,
+                     b.parity, c.err, t.y, limit_blocks(c.L.E));
 }
 
 template <int P>

@@ -70,13 +70,15 @@ struct wasl_hip_server {
   int64_t graphs = 0, rounds = 0, changes = 0;
 
   // device tables of the resident set
-  Buffer euler_cases, euler_tiles, line_cases, line_tiles, limit_cases, limit_tiles, ctl_ptrs,
-      ctl_out, err_out;
+  Buffer euler_cases, euler_tiles, line_cases, line_tiles, limit_cases, limit_tiles,
+      scatter_tiles, ctl_ptrs, ctl_out, err_out;
   struct Launch {
     int kind;       // 0 Euler stage, 1 line stage, 2 limiter (troubled + scatter)
     int p;
     int tiles;
     size_t tile_offset;   // into the kind's tile buffer
+    int tiles2 = 0;       // limiter: scatter tiles
+    size_t tile_offset2 = 0;
   };
   std::vector<Launch> launches;
   std::vector<float> mx_host[4];
@@ -215,7 +217,7 @@ void rebuild(wasl_hip_server &s) {
   std::vector<whip::BatchCase> ec;
   std::vector<whip::BatchLine> lc;
   std::vector<whip::BatchLimit> mc;
-  std::vector<int2> et, lt, mt;
+  std::vector<int2> et, lt, mt, st;
   s.launches.clear();
   s.ctl_list.clear();
   s.ctl_ids.clear();
@@ -251,7 +253,7 @@ void rebuild(wasl_hip_server &s) {
       }
       s.launches.push_back({0, key.p, static_cast<int>(et.size() - off), off});
       if (!limited.empty()) {
-        const size_t moff = mt.size();
+        const size_t moff = mt.size(), soff = st.size();
         for (Request *r : limited) {
           wasl_hip_engine &g = *r->eng;
           whip::BatchLimit m{};
@@ -269,10 +271,15 @@ void rebuild(wasl_hip_server &s) {
           m.err = g.err.as<int>();
           const int idx = static_cast<int>(mc.size());
           mc.push_back(m);
-          for (int b = 0; b < (g.E + 63) / 64; ++b)
+          for (int b = 0; b < whip::limit_blocks(g.E); ++b)
             mt.push_back(make_int2(idx, b));
+          for (int b = 0; b < (g.E + 63) / 64; ++b)
+            st.push_back(make_int2(idx, b));
         }
-        s.launches.push_back({2, key.p, static_cast<int>(mt.size() - moff), moff});
+        wasl_hip_server::Launch l{2, key.p, static_cast<int>(mt.size() - moff), moff};
+        l.tiles2 = static_cast<int>(st.size() - soff);
+        l.tile_offset2 = soff;
+        s.launches.push_back(l);
       }
     } else {
       const size_t off = lt.size();
@@ -300,6 +307,7 @@ void rebuild(wasl_hip_server &s) {
   put_table(s.line_tiles, lt, s.stream);
   put_table(s.limit_cases, mc, s.stream);
   put_table(s.limit_tiles, mt, s.stream);
+  put_table(s.scatter_tiles, st, s.stream);
   put_table(s.ctl_ptrs, s.ctl_list, s.stream);
   const size_t n = s.ctl_list.size();
   if (s.slots_capacity < n) {
@@ -347,8 +355,9 @@ template <int P> void launch_limit_class(wasl_hip_server &s, const wasl_hip_serv
   const int2 *tiles = s.limit_tiles.as<int2>() + l.tile_offset;
   hipLaunchKernelGGL((whip::limit_troubled_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream, cases,
                      tiles, b);
-  hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream,
-                     cases, tiles, b, s.mxtab[P].as<float>());
+  hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(l.tiles2), dim3(64), 0, s.stream,
+                     cases, s.scatter_tiles.as<int2>() + l.tile_offset2, b,
+                     s.mxtab[P].as<float>());
 }
 
 template <typename F> void by_order(int p, F &&f) {
