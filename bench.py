@@ -30,6 +30,18 @@ METRIC = "Euler DG DOF-updates/s (P2, 32k tris) + p50/p99 solve latency, mixed v
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch of the stage kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (one pass each) and scaled by what the same
+# counters read for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0); keyed by
+# (kernel, order, triangles, precision). The commands and counter values are in
+# profiles/euler_stage_p2_32k_fp64.md and profiles/euler_stage_mx_p3_524k.md.
+# Other workloads report null.
+MEASURED_TRAFFIC = {
+    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7886.73 * 2.0 + 6149.39) * 1024.0,
+    ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7953.21 * 2.0 + 6146.72) * 1024.0,
+    ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (230474.0 * 2.0 + 262229.0) * 1024.0,
+}
+
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -96,7 +108,7 @@ def cpu_baseline(mesh, order, cfl, budget_s: float):
     }
 
 
-def serving_measurement(count: int, seed: int) -> dict:
+def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
     and time to solution under Poisson arrivals at 80 % of the served
@@ -124,17 +136,18 @@ def serving_measurement(count: int, seed: int) -> dict:
     log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s"
         f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
 
-    done, makespan = WL.run_served(mix, meshes)
+    done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch)
     served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
     served_rate = served_updates / makespan
     capacity = count / makespan
     log(f"served closed-loop: {makespan:.2f} s, {served_rate:.4e} DOF-updates/s, "
-        f"x{served_rate / plain_rate:.2f} plain")
+        f"x{served_rate / plain_rate:.2f} plain; {stats}")
 
     rng = np.random.default_rng(seed + 1)
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
-    done_o, makespan_o = WL.run_served(mix, meshes, arrivals=arrivals)
+    done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals,
+                                          steps_per_batch=steps_per_batch)
     tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
     plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
 
@@ -161,7 +174,9 @@ def serving_measurement(count: int, seed: int) -> dict:
         "plain_dof_updates_per_s": plain_rate,
         "served_dof_updates_per_s": served_rate,
         "served_vs_plain": served_rate / plain_rate,
-        "served_precision": "mixed: FP32 matrix-core contractions, FP64 state/RK/limiter",
+        "served_precision": "unlimited 2D Euler P>=2: mixed (FP32 matrix-core contractions, "
+                            "FP64 state and RK); P<=1 and limited cases: FP64 stage; limiter, "
+                            "CFL and 1D advection FP64",
         "capacity_requests_per_s": capacity,
         "arrival_rate_per_s": rate,
         "tts_p50_s": WL.percentile(tts, 50),
@@ -169,6 +184,7 @@ def serving_measurement(count: int, seed: int) -> dict:
         "plain_tts_p50_s": WL.percentile(plain_tts, 50),
         "plain_tts_p99_s": WL.percentile(plain_tts, 99),
         "open_loop_makespan_s": makespan_o,
+        "closed_loop_server": stats,
         "density_linf_served_vs_fp64": worst,
         "density_linf_note": "max over requests of each kind of |rho coefficients, served "
                              "(mixed) - plain (FP64)|; limited rarefactions can flip a "
@@ -196,6 +212,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests of the serving mix (0: skip the serving measurement)")
     ap.add_argument("--serving-seed", type=int, default=0x5EED)
+    ap.add_argument("--serving-steps", type=int, default=16,
+                    help="SSP-RK3 steps per scheduling round of the server (even)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -270,6 +288,11 @@ def main(argv=None) -> int:
 
     U = eng.state
     finite = bool(np.isfinite(U).all())
+    kernel = ("whip::euler_stage_mx" if args.precision != "fp64" else
+              ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1])
+    tile = (f"<{p}>" if args.precision != "fp64" else
+            f"<{p},{eng.lanes}>" if eng.stage_kernel == 1 else f"<{p},{eng.lanes // 4}>")
+    traffic = MEASURED_TRAFFIC.get((kernel + tile, p, E, args.precision))
 
     result = {
         "metric": METRIC,
@@ -301,9 +324,9 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": ("whip::euler_stage_mx" if args.precision != "fp64" else
-                       ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1]),
+            "traffic": traffic,
+            "traffic_frac_of_algorithmic": None if traffic is None else traffic / bytes_stage,
+            "kernel": kernel + tile,
             "stage_kernel_us": stage_ms * 1e3,
             "algorithmic_bytes_per_launch": bytes_stage,
             "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,
@@ -313,7 +336,8 @@ def main(argv=None) -> int:
     }
     if args.serving_requests > 0:
         del eng
-        result["serving"] = serving_measurement(args.serving_requests, args.serving_seed)
+        result["serving"] = serving_measurement(args.serving_requests, args.serving_seed,
+                                                args.serving_steps)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
         result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)

@@ -7,10 +7,10 @@
  * each on a host worker thread, and admits prepared cases while device slots
  * and device memory allow (continuous batching). Each scheduling round
  * (wasl_hip_server_pump) advances every resident case by up to steps_per_batch
- * SSP-RK3 steps with one batched kernel launch per stage and class (equation,
- * polynomial order), each case taking its own CFL dt on the device and stopping
- * at its own end time; finished cases are retired and their slots reused.
- * A captured HIP graph serves every round with the same resident set.
+ * SSP-RK3 steps with one batched kernel launch per stage and class, each case
+ * taking its own CFL dt on the device and stopping at its own end time;
+ * finished cases are retired and their slots reused. A captured HIP graph per
+ * class serves every round until the class's membership changes.
  *
  * A case computes exactly what it computes alone: tiles of the batched
  * kernels are the blocks of the single-case kernel, so results do not depend
@@ -18,7 +18,12 @@
  *
  * 2D Euler cases run the stage in the server's precision (WASL_HIP_MIXED by
  * default, or WASL_HIP_BF16); the limiter, the CFL speed and the state stay
- * FP64. 1D advection cases run in FP64.
+ * FP64. Two kinds of 2D Euler case run the FP64 stage instead (each is a
+ * config switch): orders up to fp64_max_order, where the matrix-core form
+ * has no speed advantage, and cases with limiting, whose positivity
+ * guarantee needs FP64 point values near vacuum. 1D advection cases run in
+ * FP64. Cases of one (dimension, order, stage precision) form a class;
+ * classes run concurrently on streams of their own.
  *
  * Threading: one thread drives a server; the server owns its worker threads.
  */
@@ -42,6 +47,8 @@ typedef struct {
   int32_t steps_per_batch;  /* steps per scheduling round, even (default 16) */
   int32_t use_graphs;       /* capture one graph per resident set (default 1) */
   int32_t setup_threads;    /* host threads preparing cases (default 4, at most 16) */
+  int32_t fp64_max_order;   /* 2D Euler orders up to this run the FP64 stage (default 1; -1 none) */
+  int32_t fp64_limited;     /* 2D Euler cases with limiting run the FP64 stage (default 1) */
 } wasl_hip_server_config;
 
 void wasl_hip_server_config_default(wasl_hip_server_config *config);
@@ -79,7 +86,9 @@ int64_t wasl_hip_server_state_size(const wasl_hip_server *server, uint64_t id);
 
 /* Counters: 0 queued, 1 preparing or prepared, 2 resident, 3 finished and
  * unclaimed, 4 device bytes reserved, 5 graphs captured, 6 rounds run,
- * 7 resident-set changes. */
+ * 7 resident-set changes, 8 microseconds spent rebuilding launch tables and
+ * graphs, 9 microseconds spent waiting for rounds to complete, 10
+ * microseconds of case preparation (summed over the setup threads). */
 int64_t wasl_hip_server_count(const wasl_hip_server *server, int what);
 
 #ifdef __cplusplus

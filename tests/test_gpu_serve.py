@@ -2,7 +2,10 @@
 
   alone vs in the mix     bit-identical final states (a tile of the batched
                           kernels computes exactly the single-case block)
-  server vs engine        bit-identical to Engine(precision=MIXED).advance
+  server vs engine        bit-identical to Engine(precision=MIXED).advance, and
+                          for the FP64 classes (P1, limited cases, or every
+                          order with fp64_max_order=3) to the FP64 engine
+                          with its autotuned tile
   few slots               every request completes with max_resident = 2
   failures stay local     a failing case does not disturb the others
 """
@@ -17,11 +20,18 @@ pytestmark = pytest.mark.gpu
 
 
 def _requests(w):
-    """A small mix: Euler P1-P3 of uneven sizes (vortex, smooth field), a
-    limited radial rarefaction and 1D advection."""
+    """A small mix: Euler P1-P3 of uneven sizes (vortex, smooth field),
+    limited radial rarefactions and 1D advection. Every launch class holds
+    at least two cases, so each case's first tile (not the launch's) must
+    lead its step control and clear its troubled-element count."""
     vm = {n: w.Mesh.from_input(K.vortex_mesh(n)) for n in (6, 9, 14)}
     rm = w.Mesh.from_input(K.rarefaction_mesh(10))
+    rm8 = w.Mesh.from_input(K.rarefaction_mesh(8))
     lm = w.Mesh.from_input(MG.jittered_line(300))
+    lm2 = w.Mesh.from_input(MG.jittered_line(170))
+    lim = dict(equation=w.EULER, ic=w.IC_RAREFACTION, params=K.RAREFACTION,
+               boundary=(w.OUTFLOW, w.OUTFLOW), limiting=True, cfl=0.1, density_floor=1e-13,
+               pressure_floor=1e-13)
     return [
         dict(mesh=vm[6], equation=w.EULER, order=1, ic=w.IC_VORTEX, params=K.VORTEX, end_steps=40),
         dict(mesh=vm[9], equation=w.EULER, order=2, ic=w.IC_VORTEX, params=K.VORTEX, end_steps=30),
@@ -34,6 +44,15 @@ def _requests(w):
              density_floor=1e-13, pressure_floor=1e-13),
         dict(mesh=lm, equation=w.SCALAR_ADVECTION, order=2, ic=w.IC_SCALAR_PULSE, params=(),
              end_steps=60, cfl=0.1),
+        dict(mesh=vm[9], equation=w.EULER, order=1, ic=w.IC_VORTEX, params=K.VORTEX,
+             end_steps=25),
+        dict(mesh=vm[6], equation=w.EULER, order=3, ic=w.IC_SMOOTH, params=K.SMOOTH,
+             end_steps=15),
+        dict(mesh=rm8, order=2, end_steps=9, **lim),
+        dict(mesh=rm, order=1, end_steps=14, **lim),
+        dict(mesh=rm8, order=1, end_steps=10, **lim),
+        dict(mesh=lm2, equation=w.SCALAR_ADVECTION, order=2, ic=w.IC_SCALAR_PULSE, params=(),
+             end_steps=35, cfl=0.1),
     ]
 
 
@@ -67,6 +86,32 @@ def test_server_matches_single_engine(gpu):
     t, s, _ = g.advance(0.15, 0.2)
     (state, ts, ss), = _run(gpu, [dict(mesh=m, equation=gpu.EULER, order=2, ic=gpu.IC_VORTEX,
                                        params=K.VORTEX, end_time=0.2)])
+    assert (ts, ss) == (t, s)
+    assert np.array_equal(state, g.state)
+
+
+@pytest.mark.parametrize("order,limited,fp64_max_order", [
+    (1, False, 1), (2, True, 1), (3, True, 1), (2, False, 3), (3, False, 3)])
+def test_fp64_classes_match_single_engine(gpu, order, limited, fp64_max_order):
+    if limited:
+        m = gpu.Mesh.from_input(K.rarefaction_mesh(10))
+        kw = dict(boundary=(gpu.OUTFLOW, gpu.OUTFLOW), limiting=True, density_floor=1e-13,
+                  pressure_floor=1e-13)
+        # a few steps: at P3 this coarse rarefaction loses a physical cell
+        # average by t = 0.02 in FP64 as well
+        ic, prm, cfl, end = gpu.IC_RAREFACTION, K.RAREFACTION, 0.1, 0.004
+    else:
+        m = gpu.Mesh.from_input(K.vortex_mesh(12))
+        kw = {}
+        ic, prm, cfl, end = gpu.IC_VORTEX, K.VORTEX, 0.15, 0.2
+    # the engine's autotuned tile: the server's FP64 classes take the tile the
+    # autotuner gives the same key
+    g = gpu.Engine(m, gpu.EULER, order, precision=gpu.FP64, wavespeed=1.0, **kw)
+    g.project(ic, prm)
+    t, s, _ = g.advance(cfl, end)
+    (state, ts, ss), = _run(gpu, [dict(mesh=m, equation=gpu.EULER, order=order, ic=ic,
+                                       params=prm, end_time=end, cfl=cfl, **kw)],
+                            fp64_max_order=fp64_max_order)
     assert (ts, ss) == (t, s)
     assert np.array_equal(state, g.state)
 

@@ -639,14 +639,14 @@ void autotune(wasl_hip_engine &g) {
     if (std::atoi(env) == 0)
       return;
   const TuneKey key{g.dim, g.num.equation, g.p, g.num.precision, size_class(g.E)};
-  {
-    std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tuned.find(key);
-    if (it != g_tuned.end()) {
-      g.stage_kernel = it->second.first;
-      g.lanes = it->second.second;
-      return;
-    }
+  // Held through the timing: engines created on several threads (the
+  // server's setup threads) tune a key once and then all take its choice.
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) {
+    g.stage_kernel = it->second.first;
+    g.lanes = it->second.second;
+    return;
   }
   // free stream (rho, u, v, p) = (1, 0.5, 0.25, 1): mode 0 only
   std::vector<double> U(g.state_size(), 0.0);
@@ -691,7 +691,6 @@ void autotune(wasl_hip_engine &g) {
   (void)hipEventDestroy(t1);
   g.stage_kernel = best.first;
   g.lanes = best.second;
-  std::lock_guard<std::mutex> lock(g_tune_mu);
   g_tuned.emplace(key, best);
 }
 

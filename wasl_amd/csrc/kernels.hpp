@@ -329,6 +329,55 @@ template <int Np> __device__ __forceinline__ C4 trace_of(const double (&c)[4][Np
   return {t[0], t[1], t[2], t[3]};
 }
 
+// Raw buffer access (gfx9 descriptor, 32-bit offsets): the array base and
+// size sit in scalar registers, a lane passes one 32-bit byte offset and a
+// row one wave-uniform byte offset, so a load or store needs no address
+// arithmetic per row. Loads past `bytes` return zero and stores past it are
+// dropped; every access here is in bounds (arrays stay below 4 GiB).
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rsrc(const void *p, size_t bytes) {
+  const unsigned n = bytes < 0xffffffffull ? static_cast<unsigned>(bytes) : 0xffffffffu;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, static_cast<int>(n),
+                                           0x00020000);
+}
+__device__ __forceinline__ double ld_f64(Rsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ int ld_i32(Rsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(int, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ float4 ld_f32x4(Rsrc r, unsigned voff, unsigned soff) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  float4 f;
+  __builtin_memcpy(&f, &q, sizeof f);
+  return f;
+}
+__device__ __forceinline__ void st_f64(double x, Rsrc r, unsigned voff, unsigned soff) {
+  using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0u, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, x), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st_f32x4(float4 x, Rsrc r, unsigned voff, unsigned soff) {
+  using B = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0u, 0u, 0));
+  B q;
+  __builtin_memcpy(&q, &x, sizeof q);
+  __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
+}
+
+// Element at a 32-bit byte offset from a wave-uniform base: the load takes
+// the scalar-base form (base in scalar registers, one 32-bit offset register
+// per lane shared by every row), with no per-load address arithmetic.
+template <typename T>
+__device__ __forceinline__ T at_off(const T *base, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// Row r of an SoA array of E columns at column i (r wave-uniform).
+template <typename T>
+__device__ __forceinline__ T at_row(const T *base, int E, int r, int i) {
+  const T *row = base + static_cast<size_t>(r) * static_cast<unsigned>(E);
+  return at_off(row, static_cast<unsigned>(i) * static_cast<unsigned>(sizeof(T)));
+}
+
 template <int Np>
 __device__ __forceinline__ void load_coefficients(const double *U, int E, int e,
                                                   double (&c)[4][Np]) {
@@ -336,13 +385,16 @@ __device__ __forceinline__ void load_coefficients(const double *U, int E, int e,
   for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int m = 0; m < Np; ++m)
-      c[v][m] = U[(v * Np + m) * E + e];
+      c[v][m] = at_row(U, E, v * Np + m, e);
 }
 
 // Two co-resident waves per SIMD: the register budget is capped at 256.
+// One workgroup of the point split: elements [block * 256 / LANES, ...) of
+// the mesh; the leader workgroup publishes the step control. tab and red are
+// the workgroup's LDS.
 template <int P, int LANES>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void euler_stage(EulerMesh mesh, StageArgs a) {
+__device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const StageArgs &a,
+                                                 const int block, const bool leader) {
   using D = Tri<P>;
   constexpr int Np = D::Np, NQ = D::NQ, NE = D::NE;
   constexpr int PER_BLOCK = 256 / LANES;
@@ -351,7 +403,7 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
 
   const int E = mesh.E;
   const int sub = threadIdx.x % LANES;
-  const int e_raw = xcd_block() * PER_BLOCK + threadIdx.x / LANES;
+  const int e_raw = block * PER_BLOCK + threadIdx.x / LANES;
   const bool valid = e_raw < E;
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
@@ -375,9 +427,9 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt, blockIdx.x == 0))
+  if (!stage_prologue(ctl, a.parity, a.stage, dt, leader))
     return;
-  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+  if (a.sensor && leader && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
   for (int i = threadIdx.x; i < D::kTable; i += 256)
@@ -559,8 +611,12 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
     const bool troubled = !(total == 0.0 || high / total <= a.threshold);
     if (a.flags)
       a.flags[e] = troubled ? 1 : 0;
-    if (troubled)
-      a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+    if (troubled) {
+      // at most one entry per element and stage (the leader clears the count)
+      const int slot = atomicAdd(&a.counters[a.stage], 1);
+      if (slot < E)
+        a.list[slot] = e;
+    }
   }
 
   if (a.want_lambda && !a.residual_only) {
@@ -591,7 +647,13 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
                   static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
   }
-  stage_epilogue(ctl, a.parity, a.stage, dt, blockIdx.x == 0);
+  stage_epilogue(ctl, a.parity, a.stage, dt, leader);
+}
+
+template <int P, int LANES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void euler_stage(EulerMesh mesh, StageArgs a) {
+  euler_stage_body<P, LANES>(mesh, a, xcd_block(), blockIdx.x == 0);
 }
 
 // ------------------------------------------- 2D Euler, variable split ----
@@ -620,23 +682,35 @@ __device__ __forceinline__ void lane_exchange_fence() {
 // the H quads when H = 2). Volume tables are wave-uniform and read through
 // the scalar cache; edge tables depend on each element's face configuration
 // and are staged in LDS.
+// LDS of one variable-split workgroup. The exchange arrays come first and
+// 16-byte aligned, so a lane's four variables move as one 128-bit access.
+template <int P> struct alignas(16) VsShared {
+  alignas(16) double xs[64][2][4][4];   // per quad: [traces or states][point][variable]
+  alignas(16) double xf[64][4][8];      // per quad: [point][F_xi 0..3, F_eta 0..3] or [point][F 0..3]
+  alignas(16) double tab[Tri<P>::kEdgeTable];
+  double red[4];
+};
+
+// One workgroup of the variable split: elements [block * 64 / H, ...); the
+// leader workgroup publishes the step control.
 template <int P, int H>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void euler_stage_vs(EulerMesh mesh, StageArgs a) {
+__device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const StageArgs a,
+                                                    const int block, const bool leader,
+                                                    VsShared<P> &sh) {
   using D = Tri<P>;
   constexpr int Np = D::Np, NE = D::NE;
   constexpr int PER_BLOCK = 64 / H;
   constexpr int NIT = (D::NQ + 4 * H - 1) / (4 * H);
-  __shared__ double tab[D::kEdgeTable];
-  __shared__ double xs[64][2][4][4];   // per quad: [traces or states][point][variable]
-  __shared__ double xf[64][4][8];      // per quad: [point][F_xi 0..3, F_eta 0..3] or [point][F 0..3]
-  __shared__ double red[4];
+  double *tab = sh.tab;
+  auto &xs = sh.xs;
+  auto &xf = sh.xf;
+  double *red = sh.red;
 
   const int E = mesh.E;
   const int v = threadIdx.x & 3;              // owned variable
   const int quad = threadIdx.x >> 2;          // quad slot in the block
   const int h = quad % H;                     // quad index within the element
-  const int e_raw = xcd_block() * PER_BLOCK + quad / H;
+  const int e_raw = block * PER_BLOCK + quad / H;
   const bool valid = e_raw < E;
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
@@ -691,9 +765,9 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
-  if (!stage_prologue(ctl, a.parity, a.stage, dt, blockIdx.x == 0))
+  if (!stage_prologue(ctl, a.parity, a.stage, dt, leader))
     return;
-  if (a.sensor && blockIdx.x == 0 && threadIdx.x == 0)
+  if (a.sensor && leader && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
 #pragma unroll
@@ -878,8 +952,12 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
     const bool troubled = !(total == 0.0 || high / total <= a.threshold);
     if (a.flags)
       a.flags[e] = troubled ? 1 : 0;
-    if (troubled)
-      a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+    if (troubled) {
+      // at most one entry per element and stage (the leader clears the count)
+      const int slot = atomicAdd(&a.counters[a.stage], 1);
+      if (slot < E)
+        a.list[slot] = e;
+    }
   }
 
   if (a.want_lambda && !a.residual_only) {
@@ -908,7 +986,14 @@ void euler_stage_vs(EulerMesh mesh, StageArgs a) {
                 static_cast<unsigned long long>(__double_as_longlong(mx)));
     }
   }
-  stage_epilogue(ctl, a.parity, a.stage, dt, blockIdx.x == 0);
+  stage_epilogue(ctl, a.parity, a.stage, dt, leader);
+}
+
+template <int P, int H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void euler_stage_vs(EulerMesh mesh, StageArgs a) {
+  __shared__ VsShared<P> sh;
+  euler_stage_vs_body<P, H>(mesh, a, xcd_block(), blockIdx.x == 0, sh);
 }
 
 // -------------------------------------- 2D Euler, mixed precision (MFMA) ----
@@ -1068,15 +1153,23 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   // the RK combination), U^n of the same outputs, its metrics, the record of
   // edge `grp`, the neighbour's trace on that edge (64 contiguous bytes) and
   // this thread's share of the operand table.
+  // State rows (v Np + r) are wave-uniform byte offsets; a lane adds its
+  // row group's first mode (4 grp) and element (padded modes read row r of
+  // the element and are zeroed).
+  const unsigned Eu = static_cast<unsigned>(E);
+  const size_t state_bytes = static_cast<size_t>(4 * Np) * Eu * 8;
+  const Rsrc rin = rsrc(a.uin, state_bytes);
+  const unsigned eoff = static_cast<unsigned>(e) * 8u;
+  const unsigned goff = static_cast<unsigned>(4 * grp) * Eu * 8u + eoff;
   double c64[4][4], un64[4][4];
   float cf[4][4];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int m = 4 * grp + r;
-      const int idx = (v * Np + (m < Np ? m : 0)) * E + e;
-      c64[v][r] = m < Np ? a.uin[idx] : 0.0;
+      const bool real = 4 * grp + r < Np;
+      const double x = ld_f64(rin, real ? goff : eoff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
+      c64[v][r] = real ? x : 0.0;
       cf[v][r] = static_cast<float>(c64[v][r]);
     }
   constexpr int kPer = (kStaged + 255) / 256;
@@ -1086,16 +1179,21 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     const int idx = static_cast<int>(threadIdx.x) + 256 * i;
     tpre[i] = idx < kStaged ? mxtab[idx] : 0.0f;
   }
-  const double invdet = mesh.geo[4 * E + e];
-  const float gA = static_cast<float>(mesh.geo[e]), gB = static_cast<float>(mesh.geo[E + e]),
-              gC = static_cast<float>(mesh.geo[2 * E + e]),
-              gD = static_cast<float>(mesh.geo[3 * E + e]);
+  const Rsrc rgeo = rsrc(mesh.geo, static_cast<size_t>(5) * Eu * 8);
+  const double invdet = ld_f64(rgeo, eoff, 4u * Eu * 8u);
+  const float gA = static_cast<float>(ld_f64(rgeo, eoff, 0u)),
+              gB = static_cast<float>(ld_f64(rgeo, eoff, Eu * 8u)),
+              gC = static_cast<float>(ld_f64(rgeo, eoff, 2u * Eu * 8u)),
+              gD = static_cast<float>(ld_f64(rgeo, eoff, 3u * Eu * 8u));
   const int k = grp < 3 ? grp : 0;   // row group 3 idles on edge 0
-  const int meta = mesh.meta[k * E + e];
-  const int nbr = mesh.nbr[k * E + e];
-  const float nx = static_cast<float>(mesh.fn[(3 * k) * E + e]);
-  const float ny = static_cast<float>(mesh.fn[(3 * k + 1) * E + e]);
-  const float scale = static_cast<float>(mesh.fn[(3 * k + 2) * E + e]);
+  const unsigned koff = static_cast<unsigned>(k) * Eu + static_cast<unsigned>(e);
+  const int meta = ld_i32(rsrc(mesh.meta, static_cast<size_t>(3) * Eu * 4), koff * 4u, 0u);
+  const int nbr = ld_i32(rsrc(mesh.nbr, static_cast<size_t>(3) * Eu * 4), koff * 4u, 0u);
+  const Rsrc rfn = rsrc(mesh.fn, static_cast<size_t>(9) * Eu * 8);
+  const unsigned foff = (static_cast<unsigned>(3 * k) * Eu + static_cast<unsigned>(e)) * 8u;
+  const float nx = static_cast<float>(ld_f64(rfn, foff, 0u));
+  const float ny = static_cast<float>(ld_f64(rfn, foff, Eu * 8u));
+  const float scale = static_cast<float>(ld_f64(rfn, foff, 2u * Eu * 8u));
   const bool boundary = meta & kMetaBoundary;
   const bool is_left = meta & kMetaLeft;
   const bool rev = meta & kMetaReverse;
@@ -1103,11 +1201,13 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
   float4 ntr[4];
   {
-    const float4 *src = reinterpret_cast<const float4 *>(
-        a.tin + (static_cast<size_t>(boundary ? e : nbr) * 3 + (boundary ? 0 : kn)) * 16);
+    const Rsrc rt = rsrc(a.tin, static_cast<size_t>(Eu) * 3 * 64);
+    const unsigned toff =
+        (static_cast<unsigned>(boundary ? e : nbr) * 3u + static_cast<unsigned>(boundary ? 0 : kn)) *
+        64u;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-      ntr[s] = src[s];
+      ntr[s] = ld_f32x4(rt, toff, 16u * s);
   }
 
   RunCtl *ctl = a.ctl;
@@ -1206,13 +1306,24 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
 
   // U^n of the owned outputs (stages 2 and 3): requested here, while the
   // faces are evaluated, rather than held through the volume phase.
+  if (a.ca != 0.0) {
+    const Rsrc run = rsrc(a.un, state_bytes);
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
+    for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = 4 * grp + r;
-      un64[v][r] = m < Np && a.ca != 0.0 ? a.un[(v * Np + m) * E + e] : 0.0;
-    }
+      for (int r = 0; r < 4; ++r) {
+        const bool real = 4 * grp + r < Np;
+        const double x =
+            ld_f64(run, real ? goff : eoff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
+        un64[v][r] = real ? x : 0.0;
+      }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        un64[v][r] = 0.0;
+  }
 
   // ---- faces: own traces (rows 4k + s) on the matrix cores, bit-identical to
   // the traces this element's neighbours read; HLLC once per face point and
@@ -1240,13 +1351,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const int slot = meta_slot(meta);
       const int d = mesh.bdonor[slot * NE + s];
       const double *db = mesh.bbasis + (slot * NE + s) * Np;
+      const unsigned doff = static_cast<unsigned>(d) * 8u;
       float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int m = 0; m < Np; ++m) {
         const float bm = static_cast<float>(db[m]);
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          x[v] += static_cast<float>(a.uin[(v * Np + m) * E + d]) * bm;
+          x[v] += static_cast<float>(ld_f64(rin, doff, static_cast<unsigned>(v * Np + m) * Eu * 8u)) *
+                  bm;
       }
       oth = {x[0], x[1], x[2], x[3]};
     }
@@ -1269,6 +1382,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   float yf[4][4];
   double avg[4] = {0.0, 0.0, 0.0, 0.0};
   double total = 0.0, high = 0.0;
+  const Rsrc rout = rsrc(a.out, state_bytes);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
 #pragma unroll
@@ -1277,7 +1391,6 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       yf[v][r] = 0.0f;
       if (m < Np) {
         const double res = static_cast<double>(R[v][r]) * invdet;
-        const int idx = (v * Np + m) * E + e;
         double x;
         if (a.residual_only) {
           x = res;
@@ -1287,7 +1400,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
             x += a.ca * un64[v][r];
         }
         if (valid)
-          a.out[idx] = x;
+          st_f64(x, rout, goff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
         yf[v][r] = static_cast<float>(x);
         if (r == 0)
           avg[v] = x;
@@ -1315,10 +1428,11 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
         Tn[v] = mfma4(af, yf[v][s], Tn[v]);
     }
     if (valid && grp < 3) {
-      float4 *dst = reinterpret_cast<float4 *>(a.tout + (static_cast<size_t>(e) * 3 + grp) * 16);
+      const Rsrc rt = rsrc(a.tout, static_cast<size_t>(Eu) * 3 * 64);
+      const unsigned toff = (static_cast<unsigned>(e) * 3u + static_cast<unsigned>(grp)) * 64u;
 #pragma unroll
       for (int s = 0; s < 4; ++s)
-        dst[s] = make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]);
+        st_f32x4(make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]), rt, toff, 16u * s);
     }
   }
 
@@ -1333,8 +1447,12 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const bool troubled = !(total == 0.0 || high / total <= a.threshold);
       if (a.flags)
         a.flags[e] = troubled ? 1 : 0;
-      if (troubled)
-        a.list[atomicAdd(&a.counters[a.stage], 1)] = e;
+      if (troubled) {
+        // at most one entry per element and stage (the leader clears the count)
+        const int slot = atomicAdd(&a.counters[a.stage], 1);
+        if (slot < E)
+          a.list[slot] = e;
+      }
     }
   }
 
@@ -1418,7 +1536,9 @@ void euler_stage_mx(EulerMesh mesh, StageArgs a, const float *mxtab) {
 // ------------------------------------------------------------ batching ----
 //
 // Several resident cases of one class (equation, order, precision) advance in
-// one launch per stage: a tile table lists (case, block) pairs, each case
+// one launch per stage: a tile table lists (case, block) pairs (case -1:
+// padding, the workgroup exits; a table padded to a capacity keeps its launch
+// shape while cases come and go), each case
 // carries its own buffers and step control, so every case takes its own CFL
 // dt and stops at its own end time. A tile computes exactly what the
 // single-case kernel computes for that block, so a case's result does not
@@ -1438,18 +1558,24 @@ struct BatchStage {
   int stage, parity;
 };
 
-__device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b) {
+// Stage buffers and SSP-RK3 (Shu-Osher) weights of one stage: stage s reads
+// {u, w1, w2}[s] and writes {w1, w2, u}[s], out = ca U^n + cb (in + dt R).
+__device__ __forceinline__ StageArgs stage_args(double *u, double *w1, double *w2, RunCtl *ctl,
+                                                BatchStage b) {
   StageArgs a{};
-  const double ca[3] = {0.0, 0.75, 1.0 / 3.0};
-  const double cb[3] = {1.0, 0.25, 2.0 / 3.0};
-  a.uin = b.stage == 0 ? c.u : b.stage == 1 ? c.w1 : c.w2;
-  a.out = b.stage == 0 ? c.w1 : b.stage == 1 ? c.w2 : c.u;
-  a.un = c.u;
-  a.ca = ca[b.stage];
-  a.cb = cb[b.stage];
-  a.ctl = c.ctl;
+  a.uin = b.stage == 0 ? u : b.stage == 1 ? w1 : w2;
+  a.out = b.stage == 0 ? w1 : b.stage == 1 ? w2 : u;
+  a.un = u;
+  a.ca = b.stage == 0 ? 0.0 : b.stage == 1 ? 0.75 : 1.0 / 3.0;
+  a.cb = b.stage == 0 ? 1.0 : b.stage == 1 ? 0.25 : 2.0 / 3.0;
+  a.ctl = ctl;
   a.parity = b.parity;
   a.stage = b.stage;
+  return a;
+}
+
+__device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b) {
+  StageArgs a = stage_args(c.u, c.w1, c.w2, c.ctl, b);
   a.want_lambda = b.stage == 2 ? 1 : 0;
   a.sensor = c.sensor;
   a.threshold = c.threshold;
@@ -1468,8 +1594,35 @@ void euler_stage_mx_batch(const BatchCase *cases, const int2 *tiles, BatchStage 
   __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
   __shared__ double red[4];
   const int2 t = tiles[xcd_block()];
+  if (t.x < 0)
+    return;   // padding tile
   const BatchCase &c = cases[t.x];
   mx_stage<P, BF16>(c.mesh, batch_args(c, b), mxtab, t.y, t.y == 0, tab, red);
+}
+
+// FP64 point split over a batch: tiles index workgroups of 256 / LANES
+// elements, the blocks of the single-case kernel with the same tile.
+template <int P, int LANES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void euler_stage_batch(const BatchCase *cases, const int2 *tiles, BatchStage b) {
+  const int2 t = tiles[xcd_block()];
+  if (t.x < 0)
+    return;   // padding tile
+  const BatchCase &c = cases[t.x];
+  euler_stage_body<P, LANES>(c.mesh, batch_args(c, b), t.y, t.y == 0);
+}
+
+// FP64 variable split over a batch (H quads per element, 64 / H elements per
+// workgroup).
+template <int P, int H>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void euler_stage_vs_batch(const BatchCase *cases, const int2 *tiles, BatchStage b) {
+  __shared__ VsShared<P> sh;
+  const int2 t = tiles[xcd_block()];
+  if (t.x < 0)
+    return;   // padding tile
+  const BatchCase &c = cases[t.x];
+  euler_stage_vs_body<P, H>(c.mesh, batch_args(c, b), t.y, t.y == 0, sh);
 }
 
 // ------------------------------------------------- 1D scalar advection ----
@@ -1593,15 +1746,10 @@ __global__ __launch_bounds__(256) void line_stage_batch(const BatchLine *cases, 
                                                         BatchStage b) {
   __shared__ double tab[Line<P>::kTable];
   const int2 t = tiles[blockIdx.x];
+  if (t.x < 0)
+    return;   // padding tile
   const BatchLine &c = cases[t.x];
-  BatchCase shim{};
-  shim.u = c.u;
-  shim.w1 = c.w1;
-  shim.w2 = c.w2;
-  shim.ctl = c.ctl;
-  StageArgs a = batch_args(shim, b);
-  a.want_lambda = 0;
-  line_body<P>(c.mesh, a, t.y, t.y == 0, tab);
+  line_body<P>(c.mesh, stage_args(c.u, c.w1, c.w2, c.ctl, b), t.y, t.y == 0, tab);
 }
 
 // ------------------------------------------------------- reductions ----
@@ -2032,6 +2180,8 @@ template <int P>
 __global__ __launch_bounds__(64) void limit_troubled_batch(const BatchLimit *cases,
                                                            const int2 *tiles, BatchStage b) {
   const int2 t = tiles[blockIdx.x];
+  if (t.x < 0)
+    return;   // padding tile
   const BatchLimit &c = cases[t.x];
   limit_wave_body<P>(c.L, batch_out(c, b.stage), c.side, c.list, c.counters + b.stage, c.ctl,
                      b.parity, c.err, t.y, limit_blocks(c.L.E));
@@ -2042,6 +2192,8 @@ __global__ __launch_bounds__(64) void scatter_limited_batch(const BatchLimit *ca
                                                             const int2 *tiles, BatchStage b,
                                                             const float *mxtab) {
   const int2 t = tiles[blockIdx.x];
+  if (t.x < 0)
+    return;   // padding tile
   const BatchLimit &c = cases[t.x];
   scatter_body<P>(batch_out(c, b.stage), c.L.E, c.side, c.list, c.counters + b.stage, c.ctl,
                   b.parity, t.y * 64 + threadIdx.x, batch_out_traces(c, b.stage), mxtab);

@@ -52,7 +52,53 @@ size_t estimate_bytes(const whip::Mesh &m, const wasl_hip_numerics &n) {
 // Resident cases of one batched launch class.
 struct ClassKey {
   int dim, p;
-  bool operator<(const ClassKey &o) const { return dim != o.dim ? dim < o.dim : p < o.p; }
+  bool fp64;           // 2D Euler: FP64 stage kernels
+  int kernel, lanes;   // FP64 2D Euler: the engine's tile (1 point split, 2 variable split)
+  bool operator<(const ClassKey &o) const {
+    return std::tie(dim, p, fp64, kernel, lanes) <
+           std::tie(o.dim, o.p, o.fp64, o.kernel, o.lanes);
+  }
+  // elements per workgroup of the class's stage kernel
+  int block_elements() const {
+    if (dim != 2)
+      return 256;
+    if (!fp64)
+      return 64;
+    return kernel == 2 ? 64 / (lanes / 4) : 256 / lanes;
+  }
+};
+
+// One launch class: its resident cases, their launch tables and its own
+// stream. Classes hold disjoint cases, so their rounds are independent and
+// run concurrently; each round's launches of a class are captured once per
+// membership as a hipGraph.
+struct ClassRun {
+  std::vector<Request *> members;     // admission order
+  std::vector<uint64_t> ids;          // their request ids (membership test)
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  Buffer cases, tiles, limit_cases, limit_tiles, scatter_tiles;
+  // host images of the tables (alive until the next rebuild of the class)
+  std::vector<whip::BatchCase> ec;
+  std::vector<whip::BatchLine> lc;
+  std::vector<whip::BatchLimit> mc;
+  std::vector<int2> et, mt, st;
+  int ntiles = 0, nlimit = 0, nscatter = 0;   // live tiles
+  int cap = 0, cap_limit = 0, cap_scatter = 0; // launch sizes (tables padded to these)
+  hipGraphExec_t graph = nullptr;
+  std::vector<const void *> captured;          // launch arguments the graph was captured with
+
+  void release() {
+    if (graph)
+      (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    if (done)
+      (void)hipEventDestroy(done);
+    done = nullptr;
+    if (stream)
+      (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
 };
 
 }  // namespace
@@ -68,19 +114,13 @@ struct wasl_hip_server {
   std::vector<uint64_t> finished;    // done and unclaimed
   bool dirty = true;
   int64_t graphs = 0, rounds = 0, changes = 0;
+  int64_t rebuild_us = 0, wait_us = 0;
+  std::atomic<int64_t> prepare_us{0};
 
-  // device tables of the resident set
-  Buffer euler_cases, euler_tiles, line_cases, line_tiles, limit_cases, limit_tiles,
-      scatter_tiles, ctl_ptrs, ctl_out, err_out;
-  struct Launch {
-    int kind;       // 0 Euler stage, 1 line stage, 2 limiter (troubled + scatter)
-    int p;
-    int tiles;
-    size_t tile_offset;   // into the kind's tile buffer
-    int tiles2 = 0;       // limiter: scatter tiles
-    size_t tile_offset2 = 0;
-  };
-  std::vector<Launch> launches;
+  // launch classes of the resident set and the round's step-control gather
+  std::map<ClassKey, ClassRun> classes;
+  hipEvent_t round_start = nullptr;
+  Buffer ctl_ptrs, ctl_out, err_out;
   std::vector<float> mx_host[4];
   Buffer mxtab[4];
   std::vector<whip::RunCtl *> ctl_list;
@@ -88,7 +128,6 @@ struct wasl_hip_server {
   whip::StepSlot *slots_host = nullptr;   // pinned
   int *errs_host = nullptr;               // pinned
   size_t slots_capacity = 0;
-  hipGraphExec_t graph = nullptr;
 
   // workers
   std::vector<std::thread> workers;
@@ -105,8 +144,12 @@ struct wasl_hip_server {
     cv_work.notify_all();
     for (auto &t : workers)
       t.join();
-    if (graph)
-      (void)hipGraphExecDestroy(graph);
+    if (stream)
+      (void)hipStreamSynchronize(stream);
+    for (auto &kv : classes)
+      kv.second.release();
+    if (round_start)
+      (void)hipEventDestroy(round_start);
     if (slots_host)
       (void)hipHostFree(slots_host);
     if (errs_host)
@@ -162,7 +205,11 @@ void worker_loop(wasl_hip_server *s) {
       r = s->work.front();
       s->work.pop_front();
     }
+    const auto t0 = std::chrono::steady_clock::now();
     prepare(*r);
+    s->prepare_us += std::chrono::duration_cast<std::chrono::microseconds>(
+                         std::chrono::steady_clock::now() - t0)
+                         .count();
     {
       std::lock_guard<std::mutex> lock(s->mu);
       r->phase = r->status == WASL_HIP_OK ? Phase::ready : Phase::done;
@@ -197,117 +244,167 @@ void admit_ctl(wasl_hip_server &s, Request &r) {
   HIPCHECK(hipStreamSynchronize(s.stream));
 }
 
+// Host table to a device buffer, grown with headroom (its address changes
+// only when it grows).
 template <typename T> void put_table(Buffer &b, const std::vector<T> &v, hipStream_t st) {
   const size_t n = v.size() * sizeof(T);
   if (b.bytes < n) {
     size_t unused = 0;
-    b.alloc(n, unused);
+    b.alloc(2 * n, unused);
   }
   if (n)
     HIPCHECK(hipMemcpyAsync(b.p, v.data(), n, hipMemcpyHostToDevice, st));
 }
 
-// Batched launch tables of the resident set.
+// Launch tables of one class from its members (its stream is idle here).
+void rebuild_class(wasl_hip_server &s, const ClassKey &key, ClassRun &c) {
+  c.ec.clear();
+  c.lc.clear();
+  c.mc.clear();
+  c.et.clear();
+  c.mt.clear();
+  c.st.clear();
+  if (key.dim == 2) {
+    for (Request *r : c.members) {
+      wasl_hip_engine &g = *r->eng;
+      whip::BatchCase b{};
+      b.mesh = g.euler_mesh();
+      b.u = g.u.as<double>();
+      b.w1 = g.w1.as<double>();
+      b.w2 = g.w2.as<double>();
+      b.tu = g.tr_u.as<float>();
+      b.tw1 = g.tr_w1.as<float>();
+      b.tw2 = g.tr_w2.as<float>();
+      b.ctl = g.ctl.as<whip::RunCtl>();
+      const bool lim = limiting(g);
+      b.sensor = lim ? 1 : 0;
+      b.threshold = g.num.sensor_threshold;
+      b.highest_begin = g.stencil.highest_begin;
+      b.list = g.list.as<int>();
+      b.counters = g.counters.as<int>();
+      const int idx = static_cast<int>(c.ec.size());
+      c.ec.push_back(b);
+      const int per = key.block_elements();
+      for (int k = 0; k < (g.E + per - 1) / per; ++k)
+        c.et.push_back(make_int2(idx, k));
+      if (!lim)
+        continue;
+      whip::BatchLimit m{};
+      m.L = g.limiter();
+      m.u = b.u;
+      m.w1 = b.w1;
+      m.w2 = b.w2;
+      m.tu = b.tu;
+      m.tw1 = b.tw1;
+      m.tw2 = b.tw2;
+      m.side = g.side.as<double>();
+      m.list = b.list;
+      m.counters = b.counters;
+      m.ctl = b.ctl;
+      m.err = g.err.as<int>();
+      const int midx = static_cast<int>(c.mc.size());
+      c.mc.push_back(m);
+      for (int k = 0; k < whip::limit_blocks(g.E); ++k)
+        c.mt.push_back(make_int2(midx, k));
+      for (int k = 0; k < (g.E + 63) / 64; ++k)
+        c.st.push_back(make_int2(midx, k));
+    }
+  } else {
+    for (Request *r : c.members) {
+      wasl_hip_engine &g = *r->eng;
+      whip::BatchLine b{};
+      b.mesh = g.line_mesh();
+      b.u = g.u.as<double>();
+      b.w1 = g.w1.as<double>();
+      b.w2 = g.w2.as<double>();
+      b.ctl = g.ctl.as<whip::RunCtl>();
+      const int idx = static_cast<int>(c.lc.size());
+      c.lc.push_back(b);
+      for (int k = 0; k < (g.E + 255) / 256; ++k)
+        c.et.push_back(make_int2(idx, k));
+    }
+  }
+  c.ntiles = static_cast<int>(c.et.size());
+  c.nlimit = static_cast<int>(c.mt.size());
+  c.nscatter = static_cast<int>(c.st.size());
+  // launch sizes grow by doubling and never shrink: padding tiles exit at
+  // once, and a graph stays valid while its launch sizes and tables stay
+  const auto grow = [](int cap, int n) {
+    while (cap < n)
+      cap = cap ? 2 * cap : 64;
+    return cap;
+  };
+  c.cap = grow(c.cap, c.ntiles);
+  c.cap_limit = c.nlimit ? grow(c.cap_limit, c.nlimit) : 0;
+  c.cap_scatter = c.nscatter ? grow(c.cap_scatter, c.nscatter) : 0;
+  {
+    // stage kernels give each XCD a contiguous range of workgroups: spread
+    // the live tiles evenly (in order) over the padded table so every XCD
+    // gets its share
+    std::vector<int2> spread(c.cap, make_int2(-1, 0));
+    const int64_t n = c.ntiles, cap = c.cap;
+    for (int64_t j = 0; j < n; ++j)
+      spread[static_cast<size_t>(j * cap / n)] = c.et[static_cast<size_t>(j)];
+    c.et.swap(spread);
+  }
+  c.mt.resize(c.cap_limit, make_int2(-1, 0));
+  c.st.resize(c.cap_scatter, make_int2(-1, 0));
+  if (key.dim == 2)
+    put_table(c.cases, c.ec, c.stream);
+  else
+    put_table(c.cases, c.lc, c.stream);
+  put_table(c.tiles, c.et, c.stream);
+  put_table(c.limit_cases, c.mc, c.stream);
+  put_table(c.limit_tiles, c.mt, c.stream);
+  put_table(c.scatter_tiles, c.st, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  const std::vector<const void *> args = {
+      c.cases.p, c.tiles.p, c.limit_cases.p, c.limit_tiles.p, c.scatter_tiles.p,
+      reinterpret_cast<const void *>(static_cast<intptr_t>(c.cap)),
+      reinterpret_cast<const void *>(static_cast<intptr_t>(c.cap_limit)),
+      reinterpret_cast<const void *>(static_cast<intptr_t>(c.cap_scatter))};
+  if (c.graph && args != c.captured) {
+    HIPCHECK(hipGraphExecDestroy(c.graph));
+    c.graph = nullptr;
+  }
+  c.captured = args;
+}
+
+// Class membership and tables of the resident set; classes whose members
+// changed are rebuilt and lose their graph.
 void rebuild(wasl_hip_server &s) {
   std::map<ClassKey, std::vector<Request *>> cls;
   for (uint64_t id : s.resident) {
     Request *r = s.requests.at(id).get();
-    cls[{r->eng->dim, r->eng->p}].push_back(r);
+    const wasl_hip_engine &g = *r->eng;
+    const bool fp64 = !g.mixed();
+    const bool tiled = g.dim == 2 && fp64;
+    cls[{g.dim, g.p, fp64, tiled ? g.stage_kernel : 0, tiled ? g.lanes : 0}].push_back(r);
   }
-  std::vector<whip::BatchCase> ec;
-  std::vector<whip::BatchLine> lc;
-  std::vector<whip::BatchLimit> mc;
-  std::vector<int2> et, lt, mt, st;
-  s.launches.clear();
+  for (auto &[key, c] : s.classes)
+    if (!cls.count(key))
+      cls[key];   // emptied class
   s.ctl_list.clear();
   s.ctl_ids.clear();
   for (auto &[key, list] : cls) {
-    if (key.dim == 2) {
-      const size_t off = et.size();
-      std::vector<Request *> limited;
-      for (Request *r : list) {
-        wasl_hip_engine &g = *r->eng;
-        whip::BatchCase c{};
-        c.mesh = g.euler_mesh();
-        c.u = g.u.as<double>();
-        c.w1 = g.w1.as<double>();
-        c.w2 = g.w2.as<double>();
-        c.tu = g.tr_u.as<float>();
-        c.tw1 = g.tr_w1.as<float>();
-        c.tw2 = g.tr_w2.as<float>();
-        c.ctl = g.ctl.as<whip::RunCtl>();
-        const bool lim = limiting(g);
-        c.sensor = lim ? 1 : 0;
-        c.threshold = g.num.sensor_threshold;
-        c.highest_begin = g.stencil.highest_begin;
-        c.list = g.list.as<int>();
-        c.counters = g.counters.as<int>();
-        const int idx = static_cast<int>(ec.size());
-        ec.push_back(c);
-        for (int b = 0; b < (g.E + 63) / 64; ++b)
-          et.push_back(make_int2(idx, b));
-        if (lim)
-          limited.push_back(r);
-        s.ctl_list.push_back(c.ctl);
-        s.ctl_ids.push_back(r->id);
-      }
-      s.launches.push_back({0, key.p, static_cast<int>(et.size() - off), off});
-      if (!limited.empty()) {
-        const size_t moff = mt.size(), soff = st.size();
-        for (Request *r : limited) {
-          wasl_hip_engine &g = *r->eng;
-          whip::BatchLimit m{};
-          m.L = g.limiter();
-          m.u = g.u.as<double>();
-          m.w1 = g.w1.as<double>();
-          m.w2 = g.w2.as<double>();
-          m.tu = g.tr_u.as<float>();
-          m.tw1 = g.tr_w1.as<float>();
-          m.tw2 = g.tr_w2.as<float>();
-          m.side = g.side.as<double>();
-          m.list = g.list.as<int>();
-          m.counters = g.counters.as<int>();
-          m.ctl = g.ctl.as<whip::RunCtl>();
-          m.err = g.err.as<int>();
-          const int idx = static_cast<int>(mc.size());
-          mc.push_back(m);
-          for (int b = 0; b < whip::limit_blocks(g.E); ++b)
-            mt.push_back(make_int2(idx, b));
-          for (int b = 0; b < (g.E + 63) / 64; ++b)
-            st.push_back(make_int2(idx, b));
-        }
-        wasl_hip_server::Launch l{2, key.p, static_cast<int>(mt.size() - moff), moff};
-        l.tiles2 = static_cast<int>(st.size() - soff);
-        l.tile_offset2 = soff;
-        s.launches.push_back(l);
-      }
-    } else {
-      const size_t off = lt.size();
-      for (Request *r : list) {
-        wasl_hip_engine &g = *r->eng;
-        whip::BatchLine c{};
-        c.mesh = g.line_mesh();
-        c.u = g.u.as<double>();
-        c.w1 = g.w1.as<double>();
-        c.w2 = g.w2.as<double>();
-        c.ctl = g.ctl.as<whip::RunCtl>();
-        const int idx = static_cast<int>(lc.size());
-        lc.push_back(c);
-        for (int b = 0; b < (g.E + 255) / 256; ++b)
-          lt.push_back(make_int2(idx, b));
-        s.ctl_list.push_back(c.ctl);
-        s.ctl_ids.push_back(r->id);
-      }
-      s.launches.push_back({1, key.p, static_cast<int>(lt.size() - off), off});
+    ClassRun &c = s.classes[key];
+    if (!c.stream) {
+      HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      HIPCHECK(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+    }
+    std::vector<uint64_t> ids;
+    for (Request *r : list)
+      ids.push_back(r->id);
+    if (ids != c.ids) {
+      c.members = list;
+      c.ids = ids;
+      rebuild_class(s, key, c);
+    }
+    for (Request *r : c.members) {
+      s.ctl_list.push_back(r->eng->ctl.as<whip::RunCtl>());
+      s.ctl_ids.push_back(r->id);
     }
   }
-  put_table(s.euler_cases, ec, s.stream);
-  put_table(s.euler_tiles, et, s.stream);
-  put_table(s.line_cases, lc, s.stream);
-  put_table(s.line_tiles, lt, s.stream);
-  put_table(s.limit_cases, mc, s.stream);
-  put_table(s.limit_tiles, mt, s.stream);
-  put_table(s.scatter_tiles, st, s.stream);
   put_table(s.ctl_ptrs, s.ctl_list, s.stream);
   const size_t n = s.ctl_list.size();
   if (s.slots_capacity < n) {
@@ -329,35 +426,49 @@ void rebuild(wasl_hip_server &s) {
   HIPCHECK(hipStreamSynchronize(s.stream));
 }
 
-template <int P> void launch_euler_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
-                                         whip::BatchStage b) {
-  const whip::BatchCase *cases = s.euler_cases.as<whip::BatchCase>();
-  const int2 *tiles = s.euler_tiles.as<int2>() + l.tile_offset;
+template <int P>
+void launch_euler_class(wasl_hip_server &s, const ClassKey &key, ClassRun &c, whip::BatchStage b) {
+  const whip::BatchCase *cases = c.cases.as<whip::BatchCase>();
+  const int2 *tiles = c.tiles.as<int2>();
   const float *t = s.mxtab[P].as<float>();
-  if (s.cfg.precision == WASL_HIP_BF16)
-    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, true>), dim3(l.tiles), dim3(256), 0,
-                       s.stream, cases, tiles, b, t);
+  // FP64: the engine's own tile, so a case computes what the engine computes
+  if (key.fp64) {
+    const dim3 grid(c.cap), block(256);
+    if (key.kernel == 2 && key.lanes == 8)
+      hipLaunchKernelGGL((whip::euler_stage_vs_batch<P, 2>), grid, block, 0, c.stream, cases,
+                         tiles, b);
+    else if (key.kernel == 2)
+      hipLaunchKernelGGL((whip::euler_stage_vs_batch<P, 1>), grid, block, 0, c.stream, cases,
+                         tiles, b);
+    else if (key.lanes == 1)
+      hipLaunchKernelGGL((whip::euler_stage_batch<P, 1>), grid, block, 0, c.stream, cases, tiles,
+                         b);
+    else if (key.lanes == 2)
+      hipLaunchKernelGGL((whip::euler_stage_batch<P, 2>), grid, block, 0, c.stream, cases, tiles,
+                         b);
+    else
+      hipLaunchKernelGGL((whip::euler_stage_batch<P, 4>), grid, block, 0, c.stream, cases, tiles,
+                         b);
+  } else if (s.cfg.precision == WASL_HIP_BF16)
+    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, true>), dim3(c.cap), dim3(256), 0,
+                       c.stream, cases, tiles, b, t);
   else
-    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, false>), dim3(l.tiles), dim3(256), 0,
-                       s.stream, cases, tiles, b, t);
+    hipLaunchKernelGGL((whip::euler_stage_mx_batch<P, false>), dim3(c.cap), dim3(256), 0,
+                       c.stream, cases, tiles, b, t);
 }
 
-template <int P> void launch_line_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
-                                        whip::BatchStage b) {
-  hipLaunchKernelGGL((whip::line_stage_batch<P>), dim3(l.tiles), dim3(256), 0, s.stream,
-                     s.line_cases.as<whip::BatchLine>(), s.line_tiles.as<int2>() + l.tile_offset,
-                     b);
+template <int P> void launch_line_class(ClassRun &c, whip::BatchStage b) {
+  hipLaunchKernelGGL((whip::line_stage_batch<P>), dim3(c.cap), dim3(256), 0, c.stream,
+                     c.cases.as<whip::BatchLine>(), c.tiles.as<int2>(), b);
 }
 
-template <int P> void launch_limit_class(wasl_hip_server &s, const wasl_hip_server::Launch &l,
-                                         whip::BatchStage b) {
-  const whip::BatchLimit *cases = s.limit_cases.as<whip::BatchLimit>();
-  const int2 *tiles = s.limit_tiles.as<int2>() + l.tile_offset;
-  hipLaunchKernelGGL((whip::limit_troubled_batch<P>), dim3(l.tiles), dim3(64), 0, s.stream, cases,
-                     tiles, b);
-  hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(l.tiles2), dim3(64), 0, s.stream,
-                     cases, s.scatter_tiles.as<int2>() + l.tile_offset2, b,
-                     s.mxtab[P].as<float>());
+template <int P> void launch_limit_class(wasl_hip_server &s, ClassRun &c, whip::BatchStage b) {
+  const whip::BatchLimit *cases = c.limit_cases.as<whip::BatchLimit>();
+  hipLaunchKernelGGL((whip::limit_troubled_batch<P>), dim3(c.cap_limit), dim3(64), 0, c.stream,
+                     cases, c.limit_tiles.as<int2>(), b);
+  hipLaunchKernelGGL((whip::scatter_limited_batch<P>), dim3(c.cap_scatter), dim3(64), 0,
+                     c.stream,
+                     cases, c.scatter_tiles.as<int2>(), b, s.mxtab[P].as<float>());
 }
 
 template <typename F> void by_order(int p, F &&f) {
@@ -369,29 +480,55 @@ template <typename F> void by_order(int p, F &&f) {
   }
 }
 
-// One round's launches: steps_per_batch steps of three stages; per stage the
-// Euler classes, then their limiters, then the line classes.
-void launch_round(wasl_hip_server &s) {
+// One round of a class on its stream: steps_per_batch steps of three stages;
+// per stage the class's stage launch, then its limiter launches.
+void launch_class_round(wasl_hip_server &s, const ClassKey &key, ClassRun &c) {
   int parity = 0;
   for (int k = 0; k < s.cfg.steps_per_batch; ++k) {
     for (int stage = 0; stage < 3; ++stage) {
       const whip::BatchStage b{stage, parity};
-      for (const auto &l : s.launches) {
-        if (l.tiles == 0)
-          continue;
-        by_order(l.p, [&](auto P) {
-          if (l.kind == 0)
-            launch_euler_class<decltype(P)::value>(s, l, b);
-          else if (l.kind == 1)
-            launch_line_class<decltype(P)::value>(s, l, b);
-          else if (decltype(P)::value > 0)
-            launch_limit_class<decltype(P)::value>(s, l, b);
-        });
-      }
+      by_order(key.p, [&](auto P) {
+        constexpr int kP = decltype(P)::value;
+        if (key.dim == 2) {
+          launch_euler_class<kP>(s, key, c, b);
+          if constexpr (kP > 0)
+            if (c.cap_limit > 0)
+              launch_limit_class<kP>(s, c, b);
+        } else {
+          launch_line_class<kP>(c, b);
+        }
+      });
     }
     parity ^= 1;
   }
   HIPCHECK(hipGetLastError());
+}
+
+// A round of every class, each on its own stream after the work queued so
+// far on the server stream, which then waits for all of them.
+void launch_round(wasl_hip_server &s) {
+  HIPCHECK(hipEventRecord(s.round_start, s.stream));
+  for (auto &[key, c] : s.classes) {
+    if (c.members.empty())
+      continue;
+    HIPCHECK(hipStreamWaitEvent(c.stream, s.round_start, 0));
+    if (s.cfg.use_graphs) {
+      if (!c.graph) {
+        hipGraph_t gr;
+        HIPCHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        launch_class_round(s, key, c);
+        HIPCHECK(hipStreamEndCapture(c.stream, &gr));
+        HIPCHECK(hipGraphInstantiate(&c.graph, gr, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphDestroy(gr));
+        ++s.graphs;
+      }
+      HIPCHECK(hipGraphLaunch(c.graph, c.stream));
+    } else {
+      launch_class_round(s, key, c);
+    }
+    HIPCHECK(hipEventRecord(c.done, c.stream));
+    HIPCHECK(hipStreamWaitEvent(s.stream, c.done, 0));
+  }
 }
 
 __global__ void gather_slots(whip::RunCtl *const *ctls, int n, whip::StepSlot *out, int *errs) {
@@ -429,6 +566,8 @@ void wasl_hip_server_config_default(wasl_hip_server_config *c) {
   c->steps_per_batch = 16;
   c->use_graphs = 1;
   c->setup_threads = 4;
+  c->fp64_max_order = 1;
+  c->fp64_limited = 1;
 }
 
 int wasl_hip_server_create(const wasl_hip_server_config *config, wasl_hip_server **out) {
@@ -449,6 +588,7 @@ int wasl_hip_server_create(const wasl_hip_server_config *config, wasl_hip_server
       throw std::runtime_error("server: max_resident must be positive");
     c.setup_threads = std::max(1, std::min(16, c.setup_threads));
     HIPCHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&s->round_start, hipEventDisableTiming));
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
     s->budget = c.memory_budget ? static_cast<size_t>(c.memory_budget)
@@ -482,10 +622,10 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
     auto r = std::make_unique<Request>();
     r->mesh = mesh->m;
     r->num = *numerics;
-    if (r->mesh.dimension == 2)
-      r->num.precision = s->cfg.precision;
-    else
-      r->num.precision = WASL_HIP_FP64;
+    const bool fp64 = r->mesh.dimension != 2 ||
+                      r->num.polynomial_order <= s->cfg.fp64_max_order ||
+                      (s->cfg.fp64_limited && r->num.limiting);
+    r->num.precision = fp64 ? WASL_HIP_FP64 : s->cfg.precision;
     r->ic = ic;
     r->params.assign(params, params + std::max<int64_t>(param_count, 0));
     r->cfl = cfl;
@@ -558,29 +698,18 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
       s->dirty = true;
     }
     if (!s->resident.empty()) {
+      using clock = std::chrono::steady_clock;
+      const auto us = [](clock::time_point a, clock::time_point b) {
+        return std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
+      };
+      const auto t_build = clock::now();
       if (s->dirty) {
-        if (s->graph) {
-          HIPCHECK(hipGraphExecDestroy(s->graph));
-          s->graph = nullptr;
-        }
         rebuild(*s);
         ++s->changes;
         s->dirty = false;
       }
-      if (s->cfg.use_graphs) {
-        if (!s->graph) {
-          hipGraph_t gr;
-          HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-          launch_round(*s);
-          HIPCHECK(hipStreamEndCapture(s->stream, &gr));
-          HIPCHECK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
-          HIPCHECK(hipGraphDestroy(gr));
-          ++s->graphs;
-        }
-        HIPCHECK(hipGraphLaunch(s->graph, s->stream));
-      } else {
-        launch_round(*s);
-      }
+      launch_round(*s);
+      s->rebuild_us += us(t_build, clock::now());
       ++s->rounds;
       // gather the step slots (parity 0 after an even round) and error codes,
       // in the order of the launch tables
@@ -593,7 +722,9 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
                               hipMemcpyDeviceToHost, s->stream));
       HIPCHECK(hipMemcpyAsync(s->errs_host, s->err_out.p, n * sizeof(int), hipMemcpyDeviceToHost,
                               s->stream));
+      const auto t_wait = clock::now();
       HIPCHECK(hipStreamSynchronize(s->stream));
+      s->wait_us += us(t_wait, clock::now());
       std::vector<uint64_t> still;
       for (int i = 0; i < n; ++i) {
         Request &r = *s->requests.at(s->ctl_ids[i]);
@@ -700,6 +831,12 @@ int64_t wasl_hip_server_count(const wasl_hip_server *s, int what) {
     return s->rounds;
   case 7:
     return s->changes;
+  case 8:
+    return s->rebuild_us;
+  case 9:
+    return s->wait_us;
+  case 10:
+    return s->prepare_us.load();
   default:
     return -1;
   }

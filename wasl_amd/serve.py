@@ -25,6 +25,8 @@ class ServerConfig(C.Structure):
         ("steps_per_batch", C.c_int32),
         ("use_graphs", C.c_int32),
         ("setup_threads", C.c_int32),
+        ("fp64_max_order", C.c_int32),
+        ("fp64_limited", C.c_int32),
     ]
 
 
@@ -63,7 +65,8 @@ class Server:
     case (final state, time, step) and raises its error if it failed."""
 
     def __init__(self, precision: int = MIXED, max_resident: int = 256, memory_budget: int = 0,
-                 steps_per_batch: int = 16, use_graphs: bool = True, setup_threads: int = 4):
+                 steps_per_batch: int = 16, use_graphs: bool = True, setup_threads: int = 4,
+                 fp64_max_order: int = 1, fp64_limited: bool = True):
         lib = _lib()
         cfg = ServerConfig()
         lib.wasl_hip_server_config_default(C.byref(cfg))
@@ -73,6 +76,8 @@ class Server:
         cfg.steps_per_batch = steps_per_batch
         cfg.use_graphs = int(bool(use_graphs))
         cfg.setup_threads = setup_threads
+        cfg.fp64_max_order = fp64_max_order
+        cfg.fp64_limited = int(bool(fp64_limited))
         h = _P()
         _check(lib.wasl_hip_server_create(C.byref(cfg), C.byref(h)))
         self.h = h.value

@@ -141,7 +141,10 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
                steps_per_batch: int = 16, max_resident: int = 256):
     """Submit each request at its arrival offset (all at once when arrivals is
     None) and pump the server until every request is finished. Returns
-    (per-request (arrival, completion, steps, state) dict, makespan seconds)."""
+    (per-request (arrival, completion, steps, state) dict, makespan seconds,
+    server counters: rounds, graphs, resident-set changes, and the seconds
+    spent rebuilding launch tables and graphs, waiting for rounds and
+    preparing cases summed over the setup threads)."""
     from .serve import Server
 
     s = Server(precision=precision, steps_per_batch=steps_per_batch, max_resident=max_resident)
@@ -172,8 +175,11 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
                 state, steps = None, 0
             done[k] = (arrived, finish, steps, state)
     makespan = time.perf_counter() - t0
+    stats = {"rounds": s.count(6), "graphs": s.count(5), "resident_changes": s.count(7),
+             "rebuild_s": s.count(8) * 1e-6, "round_wait_s": s.count(9) * 1e-6,
+             "prepare_s": s.count(10) * 1e-6}
     s.close()
-    return done, makespan
+    return done, makespan, stats
 
 
 def fifo_times(arrivals, service):
