@@ -425,6 +425,24 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     load_coefficients<Np>(a.uin, E, nb0 >= 0 ? nb0 : e, nc);
   }
 
+  // Up to P1 the registers also hold the metrics and 1/detJ from here on;
+  // at P2 and P3 the point split has no room for them before the volume.
+  constexpr bool kEarlyMetrics = P <= 1;
+  double gm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if constexpr (kEarlyMetrics) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      gm[i] = at_row(mesh.geo, E, i, e);
+  }
+  // this thread's share of the reference tables, staged to LDS below
+  constexpr int kPer = (D::kTable + 255) / 256;
+  double tpre[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    tpre[i] = idx < D::kTable ? mesh.table[idx] : 0.0;
+  }
+
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
   if (!stage_prologue(ctl, a.parity, a.stage, dt, leader))
@@ -432,8 +450,12 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   if (a.sensor && leader && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
 
-  for (int i = threadIdx.x; i < D::kTable; i += 256)
-    tab[i] = mesh.table[i];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    if (idx < D::kTable)
+      tab[idx] = tpre[i];
+  }
   __syncthreads();
 
   double acc[4][Np];
@@ -446,8 +468,12 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   // Volume: sum_q (W_xi[q][m] F_xi + W_eta[q][m] F_eta) with the reference
   // fluxes F_xi = A Fx + B Fy, F_eta = C Fx + D Fy (= detJ J^-1 F).
   if constexpr (P > 0) {
-    const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
-                 gD = mesh.geo[3 * E + e];
+    if constexpr (!kEarlyMetrics) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        gm[i] = at_row(mesh.geo, E, i, e);
+    }
+    const double gA = gm[0], gB = gm[1], gC = gm[2], gD = gm[3];
 #pragma unroll 1
     for (int it = 0; it < (NQ + LANES - 1) / LANES; ++it) {
       const int q = it * LANES + sub;
@@ -470,6 +496,19 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
         }
       }
     }
+  }
+
+  // U^n of the variables this lane finalises (stages 2 and 3): up to P1 it
+  // is requested here, its latency overlapping the faces; at P2 and P3 the
+  // registers go to the faces and it is read in the epilogue.
+  double unr[4][Np];
+  if constexpr (kEarlyMetrics) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        unr[v][m] =
+            (v * LANES / 4 == sub && a.ca != 0.0) ? at_row(a.un, E, v * Np + m, e) : 0.0;
   }
 
   // Faces: each lane takes whole edges; traces on both sides use the left
@@ -562,7 +601,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
       acc[v][m] = lane_sum<LANES>(acc[v][m]);
 
   // Finalise: lane `sub` owns variables [sub*4/LANES, (sub+1)*4/LANES).
-  const double invdet = mesh.geo[4 * E + e];
+  const double invdet = kEarlyMetrics ? gm[4] : at_row(mesh.geo, E, 4, e);
   double out0[4] = {0.0, 0.0, 0.0, 0.0};
   double dens[Np];
 #pragma unroll
@@ -582,7 +621,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
       } else {
         y = a.cb * (c[v][m] + dt * r);
         if (a.ca != 0.0)
-          y += a.ca * a.un[idx];
+          y += a.ca * (kEarlyMetrics ? unr[v][m] : a.un[idx]);
       }
       if (valid)
         a.out[idx] = y;
