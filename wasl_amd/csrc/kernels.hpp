@@ -422,7 +422,6 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     const int k0 = sub < 3 ? sub : 0;
     meta0 = mesh.meta[k0 * E + e];
     nb0 = mesh.nbr[k0 * E + e];
-    load_coefficients<Np>(a.uin, E, nb0 >= 0 ? nb0 : e, nc);
   }
 
   // Up to P1 the registers also hold the metrics and 1/detJ from here on;
@@ -464,6 +463,11 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
 #pragma unroll
     for (int m = 0; m < Np; ++m)
       acc[v][m] = 0.0;
+
+  // The neighbour's coefficients (its index arrived with the first round
+  // trip): requested now, their latency overlapping the volume integral.
+  if constexpr (kPrefetch)
+    load_coefficients<Np>(a.uin, E, nb0 >= 0 ? nb0 : e, nc);
 
   // Volume: sum_q (W_xi[q][m] F_xi + W_eta[q][m] F_eta) with the reference
   // fluxes F_xi = A Fx + B Fy, F_eta = C Fx + D Fy (= detJ J^-1 F).
@@ -754,11 +758,11 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
   const int e = valid ? e_raw : E - 1;
   const double g = mesh.gamma, g1 = g - 1.0;
 
-  // Every global read of the stage is issued here, ahead of the step-control
+  // The global reads of the stage are issued here, ahead of the step-control
   // read (all addresses in bounds): own coefficients, metrics, the records
-  // and neighbour coefficients of this quad's edges, U^n, 1/detJ and this
-  // thread's share of the edge tables. The kernel then pays about two memory
-  // round trips (neighbour data waits for its index), not one per phase.
+  // of this quad's edges, U^n, 1/detJ and this thread's share of the edge
+  // tables. The neighbours' coefficients follow once their indices are in,
+  // so the second round trip overlaps the volume integral.
   double c[Np], acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -783,11 +787,6 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
     eny[i] = mesh.fn[(3 * k + 1) * E + e];
     escale[i] = mesh.fn[(3 * k + 2) * E + e];
     esrc[i] = (emeta[i] & kMetaBoundary) ? e : nb;
-    if (i < KP) {
-#pragma unroll
-      for (int m = 0; m < Np; ++m)
-        env[i][m] = a.uin[(v * Np + m) * E + esrc[i]];
-    }
   }
   double un[Np];
 #pragma unroll
@@ -816,6 +815,14 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
       tab[idx] = tpre[i];
   }
   __syncthreads();
+
+  // The neighbours' coefficients (their indices arrived with the first round
+  // trip): requested now, their latency overlapping the volume integral.
+#pragma unroll
+  for (int i = 0; i < KP; ++i)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      env[i][m] = at_row(a.uin, E, v * Np + m, esrc[i]);
 
   static_assert(D::kVTable <= kVTableMax, "volume table slot too small");
   const double *vt = g_vtable[P];
@@ -1185,13 +1192,12 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const int e = valid ? e_raw : E - 1;
   const float g = static_cast<float>(mesh.gamma), g1 = g - 1.0f;
 
-  // Every global read is issued here, ahead of the step-control read, so the
-  // kernel pays about two memory round trips (the neighbour's trace waits for
-  // its index) instead of one per phase; all addresses are in bounds. The
-  // element's own modes 4 grp .. 4 grp + 3 of each variable (FP64, kept for
-  // the RK combination), U^n of the same outputs, its metrics, the record of
-  // edge `grp`, the neighbour's trace on that edge (64 contiguous bytes) and
-  // this thread's share of the operand table.
+  // The global reads are issued here, ahead of the step-control read; all
+  // addresses are in bounds. The element's own modes 4 grp .. 4 grp + 3 of
+  // each variable (FP64, kept for the RK combination), its metrics, the
+  // record of edge `grp` and this thread's share of the operand table. The
+  // neighbour's trace follows once its index is in, so the second round
+  // trip overlaps the volume products.
   // State rows (v Np + r) are wave-uniform byte offsets; a lane adds its
   // row group's first mode (4 grp) and element (padded modes read row r of
   // the element and are zeroed).
@@ -1238,16 +1244,6 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const bool rev = meta & kMetaReverse;
   const int kl = meta_kl(meta);
   const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
-  float4 ntr[4];
-  {
-    const Rsrc rt = rsrc(a.tin, static_cast<size_t>(Eu) * 3 * 64);
-    const unsigned toff =
-        (static_cast<unsigned>(boundary ? e : nbr) * 3u + static_cast<unsigned>(boundary ? 0 : kn)) *
-        64u;
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-      ntr[s] = ld_f32x4(rt, toff, 16u * s);
-  }
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
@@ -1263,6 +1259,20 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       tab[idx] = tpre[i];
   }
   __syncthreads();
+
+  // The neighbour's trace on edge `grp` (64 contiguous bytes; its index
+  // arrived with the first round trip): requested now, its latency
+  // overlapping the volume products.
+  float4 ntr[4];
+  {
+    const Rsrc rt = rsrc(a.tin, static_cast<size_t>(Eu) * 3 * 64);
+    const unsigned toff =
+        (static_cast<unsigned>(boundary ? e : nbr) * 3u + static_cast<unsigned>(boundary ? 0 : kn)) *
+        64u;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      ntr[s] = ld_f32x4(rt, toff, 16u * s);
+  }
 
   mx_f4 R[4];
 #pragma unroll
