@@ -363,6 +363,14 @@ __device__ __forceinline__ void st_f32x4(float4 x, Rsrc r, unsigned voff, unsign
   __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
 }
 
+// 16 bytes per lane from global memory straight into LDS (gfx950
+// global_load_lds_dwordx4): lane l's bytes land at dst + 16 l, with dst
+// wave-uniform and 16-byte aligned. The data arrive asynchronously and are
+// counted as vector-memory loads of the issuing wave.
+__device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
+  __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+}
+
 // Element at a 32-bit byte offset from a wave-uniform base: the load takes
 // the scalar-base form (base in scalar registers, one 32-bit offset register
 // per lane shared by every row), with no per-load address arithmetic.
@@ -400,6 +408,11 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   constexpr int PER_BLOCK = 256 / LANES;
   __shared__ double tab[D::kTable];
   __shared__ double red[4];
+  // From P2 on, U^n of each wave's elements streams into LDS during the
+  // volume and face work ([row][element] per wave; rows v Np + m).
+  constexpr int EPW = 64 / LANES;   // elements per wave
+  constexpr bool kUnLds = P >= 2 && LANES >= 2 && (4 * Np * EPW * 8) % 1024 == 0;
+  __shared__ __align__(16) double unl[kUnLds ? 4 : 1][kUnLds ? 4 * Np * EPW : 1];
 
   const int E = mesh.E;
   const int sub = threadIdx.x % LANES;
@@ -448,6 +461,25 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     return;
   if (a.sensor && leader && threadIdx.x == 0)
     a.counters[(a.stage + 1) % 3] = 0;
+
+  // Each wave copies its own slab (16 bytes per lane: two elements of one
+  // row), so only the wave itself waits for it. Rows start 16-byte aligned
+  // when E is even; lanes past the last element copy nothing.
+  const bool un_lds = kUnLds && a.ca != 0.0 && (E % 2) == 0;
+  const int wv = static_cast<int>(threadIdx.x) >> 6;
+  if (un_lds) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int e0 = block * PER_BLOCK + wv * EPW;
+    constexpr int kChunks = 4 * Np * EPW * 8 / 1024;
+#pragma unroll
+    for (int i = 0; i < kChunks; ++i) {
+      const int idx = 64 * i + lane;   // 16-byte unit of the slab
+      const int row = idx / (EPW / 2), el = e0 + 2 * (idx % (EPW / 2));
+      if (el < E)
+        lds_copy16(a.un + static_cast<size_t>(row) * static_cast<unsigned>(E) + el,
+                   &unl[wv][128 * i]);
+    }
+  }
 
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
@@ -604,6 +636,12 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     for (int m = 0; m < Np; ++m)
       acc[v][m] = lane_sum<LANES>(acc[v][m]);
 
+  if (un_lds) {
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's slab has landed in LDS
+    __asm__ volatile("" ::: "memory");
+  }
+  const int el_w = static_cast<int>(threadIdx.x) / LANES - wv * EPW;   // element within the wave
+
   // Finalise: lane `sub` owns variables [sub*4/LANES, (sub+1)*4/LANES).
   const double invdet = kEarlyMetrics ? gm[4] : at_row(mesh.geo, E, 4, e);
   double out0[4] = {0.0, 0.0, 0.0, 0.0};
@@ -625,7 +663,9 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
       } else {
         y = a.cb * (c[v][m] + dt * r);
         if (a.ca != 0.0)
-          y += a.ca * (kEarlyMetrics ? unr[v][m] : a.un[idx]);
+          y += a.ca * (kEarlyMetrics ? unr[v][m]
+                       : un_lds   ? unl[kUnLds ? wv : 0][kUnLds ? (v * Np + m) * EPW + el_w : 0]
+                                  : a.un[idx]);
       }
       if (valid)
         a.out[idx] = y;
