@@ -37,8 +37,8 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/euler_stage_p2_32k_fp64.md and profiles/euler_stage_mx_p3_524k.md.
 # Other workloads report null.
 MEASURED_TRAFFIC = {
-    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7886.73 * 2.0 + 6149.39) * 1024.0,
-    ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7953.21 * 2.0 + 6146.72) * 1024.0,
+    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7895.22 * 2.0 + 6149.39) * 1024.0,
+    ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7965.57 * 2.0 + 6146.72) * 1024.0,
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (230474.0 * 2.0 + 262229.0) * 1024.0,
 }
 
