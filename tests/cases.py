@@ -48,6 +48,23 @@ def rarefaction_mesh(n=10, seed=MG.DEFAULT_SEED):
     return MG.jittered_triangles(n, (-1.0, -1.0), (1.0, 1.0), 0.15, seed)
 
 
+def odd_mesh(n=9, seed=MG.DEFAULT_SEED):
+    """rarefaction_mesh(n) with one boundary triangle split at the midpoint of
+    its edge on y = -1: 2 n^2 + 1 triangles, an odd count (transmissive
+    boundaries only)."""
+    m = rarefaction_mesh(n, seed)
+    tags, xy, tris = m.node_tags, m.node_xy, m.triangles.copy()
+    pos = {int(t): k for k, t in enumerate(tags)}
+    t = tris[0]
+    a, b = int(t[0]), int(t[1])                  # quad (0, 0): edge ll-lr lies on y = -1
+    assert xy[pos[a], 1] == -1.0 and xy[pos[b], 1] == -1.0
+    mid_tag = int(tags.max()) + 1
+    mid = 0.5 * (xy[pos[a]] + xy[pos[b]])
+    tris = np.vstack([tris, [[mid_tag, b, int(t[2])]]])
+    tris[0] = [a, mid_tag, int(t[2])]
+    return MG.TriangleMeshInput(np.append(tags, mid_tag), np.vstack([xy, mid]), tris, m.box)
+
+
 def oracle_tri(m, order, boundary=O.PERIODIC, kind="port"):
     return O.tri_euler(m.node_tags, m.node_xy, m.triangles, order, boundary=boundary, kind=kind)
 

@@ -69,7 +69,7 @@ def test_stable_dt(gpu, order):
     assert abs(g.stable_time_step(0.15) - o.stable_dt(U, 0.15)) <= 1e-15 * o.stable_dt(U, 0.15)
 
 
-@pytest.mark.parametrize("tile", [(1, 4), (2, 4), (2, 8)])
+@pytest.mark.parametrize("tile", [(1, 2), (1, 4), (2, 4), (2, 8)])
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_step_unlimited(gpu, order, tile):
     m = K.vortex_mesh(10)
@@ -83,6 +83,26 @@ def test_step_unlimited(gpu, order, tile):
         g.step(dt)
         err = K.rel_inf_per_variable(g.state, U, 4)
         print(f"P{order} step {k + 1} rel L-inf {err:.3e}")
+        assert err <= K.STEP_TOL
+
+
+@pytest.mark.parametrize("tile", [(1, 2), (1, 4), (2, 4)])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_step_odd_element_count(gpu, order, tile):
+    """An odd element count: U^n rows do not start 16-byte aligned, so the
+    stage reads U^n from global memory instead of streaming it into LDS."""
+    m = K.odd_mesh(9)
+    assert m.element_count % 2 == 1
+    o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
+    g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW, stage_kernel=tile[0], lanes=tile[1])
+    U = o.project(O.IC_SMOOTH, K.SMOOTH)
+    dt = o.stable_dt(U, 0.1)
+    g.state = U
+    for k in range(3):
+        U = o.step(U, dt)
+        g.step(dt)
+        err = K.rel_inf_per_variable(g.state, U, 4)
+        print(f"P{order} tile={tile} odd E={m.element_count} step {k + 1} rel L-inf {err:.3e}")
         assert err <= K.STEP_TOL
 
 
