@@ -108,6 +108,82 @@ def cpu_baseline(mesh, order, cfl, budget_s: float):
     }
 
 
+def precision_error(mesh, p: int, precision: int, steps: int) -> dict:
+    """The reduced-precision stage against the FP64 engine (the plain path,
+    parity-tested against the CPU oracle) on the bench workload: both start
+    from the same projected vortex and take `steps` steps of the same fixed
+    dt (the FP64 CFL step of the initial state). Density is compared as cell
+    averages (L1 weighted by element area) and as all modal coefficients
+    (L-inf), relative to the initial density's maximum cell average."""
+    import wasl_amd
+
+    runs = {}
+    for name, prec in (("fp64", wasl_amd.FP64), ("reduced", precision)):
+        g = wasl_amd.Engine(mesh, wasl_amd.EULER, p, precision=prec)
+        g.project(wasl_amd.IC_VORTEX, (5.0, 5.0, 5.0))
+        if name == "fp64":
+            dt = g.stable_time_step(0.15)
+            rho0 = g.state.reshape(4, -1, g.E)[0, 0] / np.sqrt(2.0)
+        for _ in range(steps):
+            g.step(dt)
+        runs[name] = g.state.reshape(4, -1, g.E)[0]
+        E = g.E
+        del g
+    d = runs["reduced"] - runs["fp64"]
+    avg = d[0] / np.sqrt(2.0)
+    scale = float(np.max(np.abs(rho0)))
+    detj = mesh.prepared()["element_metrics"][:E, 4]   # proportional to the element's area
+    return {
+        "steps": steps, "dt": dt,
+        "density_l1": float(np.sum(np.abs(avg) * detj) / np.sum(detj)),
+        "density_linf": float(np.max(np.abs(d))),
+        "density_linf_relative": float(np.max(np.abs(d))) / scale,
+        "reference": "FP64 engine, same initial state and fixed dt",
+    }
+
+
+def rarefaction_run(cells: int, end_time: float) -> dict:
+    """The radial double rarefaction (rho = 1, p = 0.4, velocity 2 r_hat) on
+    [-1,1]^2 with transmissive boundaries, 2 cells^2 jittered triangles, P2,
+    characteristic WENO and positivity limiting (floors 1e-13), CFL 0.1, run
+    to end_time through the FP64 engine. Reports the run's DOF-updates/s
+    (setup excluded) and the smallest cell-average density and pressure at
+    the end, or the reference's stop message if a cell average turns
+    unphysical."""
+    import torch
+
+    import wasl_amd
+    from wasl_amd import meshgen as MG
+
+    mesh = wasl_amd.Mesh.from_input(MG.jittered_triangles(cells, (-1.0, -1.0), (1.0, 1.0), 0.15))
+    g = wasl_amd.Engine(mesh, wasl_amd.EULER, 2, boundary=(wasl_amd.OUTFLOW, wasl_amd.OUTFLOW),
+                        limiting=True, density_floor=1e-13, pressure_floor=1e-13)
+    g.project(wasl_amd.IC_RAREFACTION, (1.0, 0.4, 2.0))
+    g.limit()   # the reference limits the projected initial state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = {"workload": f"radial double rarefaction, P2, 2x{cells}x{cells}={g.E} jittered "
+                       f"triangles on [-1,1]^2, transmissive, char-WENO + positivity (1e-13), "
+                       f"CFL 0.1, T = {end_time}"}
+    try:
+        t, steps, _ = g.advance(0.1, end_time)
+        out["status"] = "completed"
+    except wasl_amd.SimulationError as exc:
+        t, steps = float("nan"), -1
+        out["status"] = f"stopped: {exc}"
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out.update({"elements": g.E, "steps": steps, "time": t, "wall_s": wall})
+    if steps > 0:
+        out["dof_updates_per_s"] = g.E * g.Np * 4 * steps / wall
+        U = g.state.reshape(4, g.Np, g.E)[:, 0] / np.sqrt(2.0)
+        rho, mx, my, en = U
+        p = 0.4 * (en - 0.5 * (mx * mx + my * my) / rho)
+        out["min_cell_density"] = float(rho.min())
+        out["min_cell_pressure"] = float(p.min())
+    return out
+
+
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
@@ -212,6 +288,12 @@ def main(argv=None) -> int:
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests of the serving mix (0: skip the serving measurement)")
     ap.add_argument("--serving-seed", type=int, default=0x5EED)
+    ap.add_argument("--workload", choices=["vortex", "rarefaction"], default="vortex",
+                    help="vortex: the headline (config 2 sizes by default); rarefaction: the "
+                         "limited radial rarefaction run to --end-time, one JSON line of its own")
+    ap.add_argument("--end-time", type=float, default=0.3)
+    ap.add_argument("--error-steps", type=int, default=20,
+                    help="steps of the reduced-precision error check (--precision mixed/bf16)")
     ap.add_argument("--serving-steps", type=int, default=16,
                     help="SSP-RK3 steps per scheduling round of the server (even)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -243,6 +325,15 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
         del src, dst
+
+    if args.workload == "rarefaction":
+        res = rarefaction_run(args.cells, args.end_time)
+        log(json.dumps(res))
+        print(json.dumps({"metric": "limited rarefaction DOF-updates/s (P2)",
+                          "value": res.get("dof_updates_per_s"), "unit": "DOF-updates/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic", "config": res}), flush=True)
+        return 0
 
     p = args.order
     mesh_in = MG.jittered_triangles(args.cells, (0.0, 0.0), (10.0, 10.0), 0.15)
@@ -334,6 +425,11 @@ def main(argv=None) -> int:
         },
         "cpu_baseline": None,
     }
+    if args.precision != "fp64":
+        result["precision_error"] = precision_error(mesh, p, precision, args.error_steps)
+        log(f"{args.precision} vs FP64 after {args.error_steps} steps: density L1 "
+            f"{result['precision_error']['density_l1']:.3e}, L-inf "
+            f"{result['precision_error']['density_linf']:.3e}")
     if args.serving_requests > 0:
         del eng
         result["serving"] = serving_measurement(args.serving_requests, args.serving_seed,
