@@ -125,10 +125,11 @@ int wasl_hip_mesh_project(const wasl_hip_mesh *mesh, int polynomial_order, doubl
  * (include/wasl/case/euler_case.hpp:28-42, scalar_advection_case.hpp:29-45)
  * that reach the hot path. */
 typedef struct {
-  int32_t equation;          /* WASL_HIP_EULER (2D triangles) or WASL_HIP_SCALAR_ADVECTION (1D lines) */
+  int32_t equation;          /* WASL_HIP_EULER (2D triangles) or WASL_HIP_SCALAR_ADVECTION (1D lines,
+                                or 2D periodic triangles: scalar_advection_triangle.cpp:15-46) */
   int32_t polynomial_order;  /* 0..3 */
   double heat_capacity_ratio;
-  double wavespeed;          /* scalar advection a */
+  double wavespeed;          /* scalar advection a (1D), velocity_x (2D) */
   int32_t boundary_left;     /* 1D: per end. 2D: boundary_left applies to every boundary face
                                 (periodic box or transmissive outflow). */
   int32_t boundary_right;
@@ -143,6 +144,7 @@ typedef struct {
   int32_t stage_kernel;      /* 0: autotuned; 1: point split (1, 2 or 4 lanes per element,
                                 each lane a share of the quadrature points); 2: variable split
                                 (4 or 8 lanes per element, each lane one conserved variable) */
+  double velocity_y;         /* 2D scalar advection: y velocity (ScalarAdvectionCase::velocity_y) */
 } wasl_hip_numerics;
 
 /* Set fields to the reference defaults (config.rs:85-103). */
@@ -177,7 +179,11 @@ int wasl_hip_engine_stencil(const wasl_hip_engine *engine, int32_t *neighbors, d
  * ic 2: radial rarefaction (rho, p, speed);
  * ic 3: smooth field (L, then 4 x (a, m, n, phase));
  * ic 10: scalar pulse sin(2 pi x) + exp(-((x-0.5)/0.05)^2);
- * ic 11: scalar regions (default, sine flag, amplitude, k*3 intervals).
+ * ic 11: scalar regions (default, sine flag, amplitude, k*3 intervals);
+ * ic 12: 2D scalar circles (default, k*4 (cx, cy, r, value)),
+ *        src/case/initial_regions.cpp:28-34;
+ * ic 13: 2D scalar plane wave (L, kx, ky): sin(2 pi (kx x + ky y) / L).
+ * Codes 0-3 need an Euler engine, 10-11 a line engine, 12-13 a 2D scalar one.
  * The result is written to state_out (host) and made the resident state. */
 int wasl_hip_project(wasl_hip_engine *engine, int ic, const double *params, int64_t param_count,
                      double *state_out);
@@ -188,8 +194,10 @@ int wasl_hip_download_state(wasl_hip_engine *engine, double *state);
 /* One residual evaluation R(state) with the case's boundary exterior,
  * replacing euler_triangle_residual<hllc_flux> (dg/euler/triangle_residual.hpp:25-107)
  * with periodic_triangle_euler_exterior (src/boundaries/periodic_triangle_trace.cpp:84-105),
- * or scalar_advection_residual (dg/scalar_advection/line_residual.hpp:48-111)
- * with make_line_boundary_exterior (boundaries/line_boundary_conditions.hpp:36-57).
+ * scalar_advection_residual (dg/scalar_advection/line_residual.hpp:48-111)
+ * with make_line_boundary_exterior (boundaries/line_boundary_conditions.hpp:36-57),
+ * or triangle_scalar_advection_residual (dg/scalar_advection/triangle_residual.hpp:25-105)
+ * with periodic_triangle_exterior (src/boundaries/periodic_triangle_trace.cpp:69-82).
  * Host buffers in and out; the resident state is not touched. */
 int wasl_hip_residual(wasl_hip_engine *engine, const double *state, double *rhs);
 
@@ -198,8 +206,8 @@ int wasl_hip_residual(wasl_hip_engine *engine, const double *state, double *rhs)
  * decision (may be NULL). */
 int wasl_hip_limit(wasl_hip_engine *engine, uint8_t *flags);
 
-/* euler_cfl_timestep / advection_cfl_timestep (src/time/cfl_timestep.cpp:55-84)
- * of the resident state. */
+/* euler_cfl_timestep / advection_cfl_timestep / triangle_scalar_cfl_timestep
+ * (src/time/cfl_timestep.cpp:53-84) of the resident state. */
 int wasl_hip_stable_dt(wasl_hip_engine *engine, double cfl, double *dt);
 
 /* One SSP-RK3 step of the resident state (time/ssp_rk3.hpp:37-61), the

@@ -61,8 +61,10 @@ int wasl_hip_run_euler_case(const char *mesh_path, const char *output_path,
 
 /* run_scalar_advection_case (crates/wasl-bridge/src/bridge.cpp:132-171):
  * intervals packed 3 per interval (lower, upper, value), circles 4 per circle
- * (center_x, center_y, radius, value). Lines (dimensions = 1) only, serial:
- * mpi != 0 and dimensions = 2 (advection on triangles) fail. */
+ * (center_x, center_y, radius, value). Serial only (mpi != 0 fails). Lines
+ * (dimensions = 1) or periodic triangles (dimensions = 2,
+ * src/case/scalar_advection_triangle.cpp:15-46: every boundary periodic, the
+ * initial state from the circles, velocity (velocity_x, velocity_y)). */
 int wasl_hip_run_scalar_advection_case(
     const char *mesh_path, const char *output_path, const char *restart_path, int mpi,
     int dimensions, int polynomial_order, double velocity_x, double velocity_y, double cfl,

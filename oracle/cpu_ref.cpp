@@ -711,6 +711,7 @@ struct LineFace {
 
 struct orc_case {
   int dim = 2;
+  bool scalar = false;  // 2D scalar advection (U[element][mode]) instead of Euler
   int p = 0;
   int n = 0;  // modes
   int V = 4;  // variables
@@ -721,6 +722,7 @@ struct orc_case {
   std::vector<Face> faces;
   std::vector<int32_t> donors;
   Vec donor_basis;
+  double vx = 0.0, vy = 0.0;  // 2D scalar advection velocity
   // lines
   std::vector<LineElem> lines;
   std::vector<LineFace> lfaces;
@@ -985,8 +987,92 @@ void line_scalar_residual(const orc_case &c, const double *U, const double *ext,
   }
 }
 
+// src/boundaries/periodic_triangle_trace.cpp:69-82: exterior per [face][point]
+Vec tri_scalar_exterior(const orc_case &c, const double *U) {
+  Vec ext(c.donors.size(), 0.0);
+  for (size_t pt = 0; pt < c.donors.size(); ++pt) {
+    if (c.donors[pt] < 0)
+      continue;
+    const size_t d = static_cast<size_t>(c.donors[pt]);
+    for (int m = 0; m < c.n; ++m)
+      ext[pt] += U[d * c.n + m] * c.donor_basis[pt * c.n + m];
+  }
+  return ext;
+}
+
+// include/wasl/equations/scalar_advection/scalar_advection.hpp:45-50
+double upwind_normal(double vx, double vy, double nx, double ny, double l, double r) {
+  const double ns = vx * nx + vy * ny;
+  return ns * (ns >= 0.0 ? l : r);
+}
+
+// include/wasl/dg/scalar_advection/triangle_residual.hpp:25-105 (constant
+// velocity). The volume rule has (p+1)^2 points, two fewer orders than the
+// Euler residual's.
+void tri_scalar_residual(const orc_case &c, const double *U, const double *ext, double *rhs) {
+  const int n = c.n, P = c.p + 1;
+  const size_t E = c.E();
+  const TriRule vol = tri_rule(c.p);
+  const auto edges = edge_rules(c.p);
+  std::fill(rhs, rhs + E * n, 0.0);
+  Vec b(n), gx(n), ge(n);
+  for (size_t e = 0; e < E; ++e) {
+    const TriMap map = c.tris[e].map();
+    const double *modes = U + e * n;
+    double *r = rhs + e * n;
+    for (int q = 0; q < vol.size(); ++q) {
+      dubiner(vol.xi[q], vol.eta[q], c.p, b.data());
+      dubiner_gradient(vol.xi[q], vol.eta[q], c.p, gx.data(), ge.data());
+      const double state = modal_sum(modes, b.data(), n);
+      for (int m = 0; m < n; ++m) {
+        double dx, dy;
+        map.scaled_gradient(gx[m], ge[m], dx, dy);
+        r[m] += vol.w[q] * state * (c.vx * dx + c.vy * dy);
+      }
+    }
+  }
+  Vec lb(n), rb(n);
+  for (size_t f = 0; f < c.faces.size(); ++f) {
+    const Face &face = c.faces[f];
+    const size_t L = static_cast<size_t>(face.left);
+    const EdgeRule &rule = edges[face.left_edge];
+    const TriMap lmap = c.tris[L].map();
+    const double scale = lmap.edge_length(face.left_edge) / reference_edge_length(face.left_edge);
+    for (int q = 0; q < P; ++q) {
+      dubiner(rule.xi[q], rule.eta[q], c.p, lb.data());
+      const double ls = modal_sum(U + L * n, lb.data(), n);
+      double rs = 0.0;
+      if (!face.boundary()) {
+        double rxi, reta;
+        neighbor_edge_point(face, rule.xi[q], rule.eta[q], rxi, reta);
+        dubiner(rxi, reta, c.p, rb.data());
+        rs = modal_sum(U + static_cast<size_t>(face.right) * n, rb.data(), n);
+      } else {
+        rs = ext[f * P + q];
+      }
+      const double flux = upwind_normal(c.vx, c.vy, face.nx, face.ny, ls, rs);
+      const double w = rule.w[q] * scale;
+      for (int m = 0; m < n; ++m)
+        rhs[L * n + m] -= w * lb[m] * flux;
+      if (!face.boundary()) {
+        const size_t R = static_cast<size_t>(face.right);
+        for (int m = 0; m < n; ++m)
+          rhs[R * n + m] += w * rb[m] * flux;
+      }
+    }
+  }
+  for (size_t e = 0; e < E; ++e) {
+    const double inv = 1.0 / c.tris[e].map().jacobian_determinant();
+    for (int m = 0; m < n; ++m)
+      rhs[e * n + m] *= inv;
+  }
+}
+
 void residual(const orc_case &c, const double *U, double *rhs) {
-  if (c.dim == 2) {
+  if (c.scalar) {
+    const Vec ext = tri_scalar_exterior(c, U);
+    tri_scalar_residual(c, U, ext.data(), rhs);
+  } else if (c.dim == 2) {
     const Vec ext = tri_exterior(c, U);
     tri_euler_residual(c, U, ext.data(), rhs);
   } else {
@@ -1156,6 +1242,12 @@ double stable_dt(const orc_case &c, const double *U, double cfl) {
     const double longest = std::max({m.edge_length(0), m.edge_length(1), m.edge_length(2)});
     h = std::min(h, 4.0 * m.jacobian_determinant() / longest);
   }
+  if (c.scalar) {  // cfl_timestep.cpp:62-68: constant for the whole run
+    const double speed = std::hypot(c.vx, c.vy);
+    if (E == 0 || speed == 0.0)
+      return 0.0;
+    return cfl * h / (speed * (2.0 * c.p + 1.0));
+  }
   const double scale = 1.0 / std::sqrt(2.0);
   double speed = 0.0;
   for (size_t e = 0; e < E; ++e) {
@@ -1177,7 +1269,7 @@ void step(orc_case &c, double dt, int limiting, double thr, double rf, double pf
   c.rhs.resize(N);
   c.euler.resize(N);
   const auto post = [&](double *s) {
-    if (limiting && c.dim == 2)
+    if (limiting && c.dim == 2 && !c.scalar)
       limit(c, thr, rf, pf, s, nullptr);
   };
   residual(c, U, c.rhs.data());
@@ -1197,7 +1289,10 @@ void step(orc_case &c, double dt, int limiting, double thr, double rf, double pf
 // src/case/euler_diagnostics.cpp:30-36, scalar_diagnostics.cpp:18-25
 double mass(const orc_case &c, const double *U) {
   double r = 0.0;
-  if (c.dim == 2) {
+  if (c.scalar) {  // scalar_diagnostics.cpp:27-34
+    for (size_t e = 0; e < c.E(); ++e)
+      r += std::sqrt(2.0) * c.tris[e].map().jacobian_determinant() * U[e * c.n];
+  } else if (c.dim == 2) {
     for (size_t e = 0; e < c.E(); ++e)
       r += std::sqrt(2.0) * c.tris[e].map().jacobian_determinant() * U[e];
   } else {
@@ -1260,6 +1355,21 @@ Prim euler_ic(int ic, const double *prm, int64_t np, double gamma, double x, dou
   }
 }
 
+// src/case/initial_regions.cpp:28-34 (circles) and a plane sine wave
+double scalar_ic_2d(int ic, const double *prm, int64_t np, double x, double y) {
+  if (ic == ORC_IC_SCALAR_CIRCLES) {
+    for (int64_t k = 1; k + 4 <= np; k += 4) {
+      const double dx = x - prm[k], dy = y - prm[k + 1];
+      if (dx * dx + dy * dy <= prm[k + 2] * prm[k + 2])
+        return prm[k + 3];
+    }
+    return prm[0];
+  }
+  if (ic == ORC_IC_SCALAR_WAVE)
+    return std::sin(2.0 * kPi * (prm[1] * x + prm[2] * y) / prm[0]);
+  throw std::runtime_error("oracle: unknown 2D scalar initial condition");
+}
+
 double scalar_ic(int ic, const double *prm, int64_t np, double x) {
   if (ic == ORC_IC_SCALAR_PULSE) {
     const double s = (x - 0.5) / 0.05;
@@ -1279,6 +1389,27 @@ double scalar_ic(int ic, const double *prm, int64_t np, double x) {
 // src/case/initial_state.cpp:16-64 with dg/l2_projection.hpp:36-76
 void project(const orc_case &c, int ic, const double *prm, int64_t np, double *U) {
   const size_t E = c.E();
+  if (c.scalar) {
+    // project_triangle_field writes U[0][mode][element]; solver_coefficients
+    // transposes it to U[element][mode] (scalar_coefficient_layout.hpp).
+    const TriRule rule = tri_rule(std::max(c.p, 5));
+    Vec tmp(c.n * E, 0.0), b(c.n);
+    for (size_t e = 0; e < E; ++e) {
+      const TriMap map = c.tris[e].map();
+      for (int q = 0; q < rule.size(); ++q) {
+        double x, y;
+        map.to_physical(rule.xi[q], rule.eta[q], x, y);
+        dubiner(rule.xi[q], rule.eta[q], c.p, b.data());
+        const double value = scalar_ic_2d(ic, prm, np, x, y);
+        for (int m = 0; m < c.n; ++m)
+          tmp[static_cast<size_t>(m) * E + e] += rule.w[q] * value * b[m];
+      }
+    }
+    for (size_t e = 0; e < E; ++e)
+      for (int m = 0; m < c.n; ++m)
+        U[e * c.n + m] = tmp[static_cast<size_t>(m) * E + e];
+    return;
+  }
   if (c.dim == 2) {
     const TriRule rule = tri_rule(std::max(c.p, 5));
     std::fill(U, U + 4 * c.n * E, 0.0);
@@ -1349,6 +1480,33 @@ orc_case *orc_tri_euler_create(int64_t node_count, const int64_t *node_tags, con
                         c->faces);
         build_trace(*c);
         build_stencil(*c);
+      }) != 0) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+// src/case/scalar_advection_triangle.cpp:15-46: periodic boundaries only,
+// no limiter.
+orc_case *orc_tri_scalar_create(int64_t node_count, const int64_t *node_tags, const double *node_xy,
+                                int64_t triangle_count, const int64_t *triangle_nodes, int order,
+                                double velocity_x, double velocity_y) {
+  orc_case *c = new orc_case;
+  if (guarded([&] {
+        if (order < 0 || order > 3)
+          throw std::runtime_error("oracle: polynomial order must be 0..3");
+        c->dim = 2;
+        c->scalar = true;
+        c->p = order;
+        c->n = tri_modes(order);
+        c->V = 1;
+        c->vx = velocity_x;
+        c->vy = velocity_y;
+        c->boundary = ORC_PERIODIC;
+        build_triangles(node_count, node_tags, node_xy, triangle_count, triangle_nodes, c->tris,
+                        c->faces);
+        build_trace(*c);
       }) != 0) {
     delete c;
     return nullptr;
@@ -1499,7 +1657,7 @@ int orc_residual(const orc_case *c, const double *state, double *rhs) {
 int orc_limit(const orc_case *c, double threshold, double density_floor, double pressure_floor,
               double *state, uint8_t *flags) {
   return guarded([&] {
-    if (c->dim != 2)
+    if (c->dim != 2 || c->scalar)
       throw std::runtime_error("oracle: the limiter is implemented for 2D Euler");
     limit(*c, threshold, density_floor, pressure_floor, state, flags);
   });

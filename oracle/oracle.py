@@ -24,7 +24,7 @@ PATHS = {
 
 PERIODIC, INFLOW, OUTFLOW = 0, 1, 2
 IC_EULER_REGIONS, IC_VORTEX, IC_RAREFACTION, IC_SMOOTH = 0, 1, 2, 3
-IC_SCALAR_PULSE, IC_SCALAR_REGIONS = 10, 11
+IC_SCALAR_PULSE, IC_SCALAR_REGIONS, IC_SCALAR_CIRCLES, IC_SCALAR_WAVE = 10, 11, 12, 13
 
 _libs: dict[str, C.CDLL] = {}
 
@@ -52,6 +52,8 @@ def load(kind: str = "port") -> C.CDLL:
     lib.orc_error.restype = C.c_char_p
     lib.orc_tri_euler_create.restype = _P
     lib.orc_tri_euler_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int]
+    lib.orc_tri_scalar_create.restype = _P
+    lib.orc_tri_scalar_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, _d]
     lib.orc_line_scalar_create.restype = _P
     lib.orc_line_scalar_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int,
                                            _d, C.c_int, _d]
@@ -303,6 +305,20 @@ def tri_euler(node_tags, node_xy, triangles, order, gamma=1.4, boundary=PERIODIC
     xy = np.ascontiguousarray(node_xy, np.float64).reshape(-1)
     tri = np.ascontiguousarray(triangles, np.int64).reshape(-1)
     h = lib.orc_tri_euler_create(tags.size, tags, xy, tri.size // 3, tri, order, gamma, boundary)
+    if not h:
+        raise OracleError(lib.orc_error().decode())
+    return Case(lib, h, 2)
+
+
+def tri_scalar(node_tags, node_xy, triangles, order, velocity_x, velocity_y,
+               kind="port") -> Case:
+    """2D scalar advection on periodic triangles, state U[element][mode]."""
+    lib = load(kind)
+    tags = np.ascontiguousarray(node_tags, np.int64)
+    xy = np.ascontiguousarray(node_xy, np.float64).reshape(-1)
+    tri = np.ascontiguousarray(triangles, np.int64).reshape(-1)
+    h = lib.orc_tri_scalar_create(tags.size, tags, xy, tri.size // 3, tri, order, velocity_x,
+                                  velocity_y)
     if not h:
         raise OracleError(lib.orc_error().decode())
     return Case(lib, h, 2)

@@ -43,7 +43,9 @@ enum {
   ORC_IC_RAREFACTION = 2,    /* params: rho, p, speed (velocity speed * r_hat, zero at origin) */
   ORC_IC_SMOOTH = 3,         /* params: L, then 4 x (a, m, n, phase): rho = 1 + 0.1 sum a sin(2pi(mx+ny)/L+phase), p=1, u=v=0.5 */
   ORC_IC_SCALAR_PULSE = 10,  /* sin(2 pi x) + exp(-((x-0.5)/0.05)^2) */
-  ORC_IC_SCALAR_REGIONS = 11 /* params: default, sine(0/1), amplitude, k*3 (lower, upper, value) */
+  ORC_IC_SCALAR_REGIONS = 11, /* params: default, sine(0/1), amplitude, k*3 (lower, upper, value) */
+  ORC_IC_SCALAR_CIRCLES = 12, /* 2D: default, k*4 (cx, cy, r, value), initial_regions.cpp:28-34 */
+  ORC_IC_SCALAR_WAVE = 13     /* 2D: L, kx, ky: sin(2 pi (kx x + ky y) / L) */
 };
 
 const char *orc_error(void);
@@ -55,6 +57,12 @@ const char *orc_error(void);
 orc_case *orc_tri_euler_create(int64_t node_count, const int64_t *node_tags, const double *node_xy,
                                int64_t triangle_count, const int64_t *triangle_nodes, int order,
                                double gamma, int boundary);
+
+/* 2D scalar advection on periodic triangles with constant velocity
+ * (scalar_advection_triangle.cpp:15-46). State U[element][mode]; no limiter. */
+orc_case *orc_tri_scalar_create(int64_t node_count, const int64_t *node_tags, const double *node_xy,
+                                int64_t triangle_count, const int64_t *triangle_nodes, int order,
+                                double velocity_x, double velocity_y);
 
 /* 1D scalar advection on a line mesh. Segments list two node tags. */
 orc_case *orc_line_scalar_create(int64_t node_count, const int64_t *node_tags,

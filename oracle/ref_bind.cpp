@@ -7,12 +7,14 @@
 // limiter, CFL, SSP-RK3, projection, diagnostics); this file only adapts
 // arrays to the reference types and composes the calls the way the
 // reference's case drivers do (src/case/euler_triangle.cpp:16-50,
-// src/case/scalar_advection_line.cpp:12-31, src/case/serial_case_runner.cpp).
+// src/case/scalar_advection_line.cpp:12-31,
+// src/case/scalar_advection_triangle.cpp:15-46, src/case/serial_case_runner.cpp).
 //
 // Two compositions have no reference counterpart and are marked:
 //   - the transmissive 2D exterior (own trace, no limiter neighbour across),
 //   - the analytic initial conditions (vortex, rarefaction, smooth field,
-//     scalar pulse), passed to the reference's projection templates.
+//     scalar pulse, 2D scalar plane wave), passed to the reference's
+//     projection templates.
 
 #include "oracle_api.h"
 
@@ -37,6 +39,7 @@
 #include "wasl/dg/l2_projection.hpp"
 #include "wasl/dg/modal_layout.hpp"
 #include "wasl/dg/scalar_advection/line_residual.hpp"
+#include "wasl/dg/scalar_advection/triangle_residual.hpp"
 #include "wasl/equations/euler/primitive.hpp"
 #include "wasl/flux/hllc.hpp"
 #include "wasl/limiting/euler_limiter_stencil.hpp"
@@ -76,6 +79,7 @@ template <typename F> int guarded(F &&f) {
 
 struct orc_case {
   int dim = 2;
+  bool tri_scalar = false;  // 2D scalar advection, U[element][mode]
   int p = 0;
   std::size_t n = 0;
   int boundary = ORC_PERIODIC;
@@ -84,6 +88,7 @@ struct orc_case {
   wasl::PeriodicTriangleTrace trace;
   wasl::LineMesh line;
   wasl::ScalarAdvection scalar{0.0};
+  wasl::ScalarAdvection2D advection{0.0, 0.0};
   wasl::LineBoundaryConditions conditions;
   wasl::ExteriorFunction exterior;
   wasl::EulerLimiterStencil stencil;
@@ -91,7 +96,7 @@ struct orc_case {
 
   std::size_t E() const { return dim == 2 ? tri.num_elements() : line.num_elements(); }
   std::size_t F() const { return dim == 2 ? tri.faces.size() : line.faces.size(); }
-  std::size_t V() const { return dim == 2 ? 4 : 1; }
+  std::size_t V() const { return dim == 2 && !tri_scalar ? 4 : 1; }
 };
 
 namespace {
@@ -123,7 +128,11 @@ std::vector<double> own_trace_exterior(const orc_case &c, const std::vector<doub
 
 void residual(const orc_case &c, const std::vector<double> &U, std::vector<double> &rhs) {
   rhs.resize(U.size());
-  if (c.dim == 2) {
+  if (c.tri_scalar) {
+    const auto ext = wasl::periodic_triangle_exterior(c.trace, U, c.n);
+    wasl::triangle_scalar_advection_residual(c.tri, c.advection, c.p, U.data(), ext.data(),
+                                             rhs.data());
+  } else if (c.dim == 2) {
     const auto ext = c.boundary == ORC_PERIODIC
                          ? wasl::periodic_triangle_euler_exterior(c.trace, U, c.E(), c.n, c.F())
                          : own_trace_exterior(c, U);
@@ -193,6 +202,26 @@ wasl::EulerPrimitive analytic_euler(int ic, const double *prm, double gamma, dou
 }
 
 void project(const orc_case &c, int ic, const double *prm, int64_t np, std::vector<double> &U) {
+  if (c.tri_scalar) {
+    if (ic == ORC_IC_SCALAR_CIRCLES) {
+      wasl::ScalarAdvectionCase config;
+      config.polynomial_order = c.p;
+      config.initial_default = prm[0];
+      for (int64_t k = 1; k + 4 <= np; k += 4)
+        config.initial_circles.push_back({prm[k], prm[k + 1], prm[k + 2], prm[k + 3]});
+      U = wasl::initial_coefficients(c.tri, config);
+      return;
+    }
+    if (ic != ORC_IC_SCALAR_WAVE)
+      throw std::runtime_error("oracle: unknown 2D scalar initial condition");
+    const auto projected = wasl::project_triangle_field(
+        c.tri, c.p, wasl::triangle_volume_rule(std::max(c.p, 5)), 1,
+        [&](double x, double y, double *v) {
+          v[0] = std::sin(2.0 * std::numbers::pi * (prm[1] * x + prm[2] * y) / prm[0]);
+        });
+    U = wasl::solver_coefficients(projected, c.E(), c.n);
+    return;
+  }
   if (c.dim == 2) {
     if (ic == ORC_IC_EULER_REGIONS) {
       wasl::EulerCase config;
@@ -243,10 +272,15 @@ void project(const orc_case &c, int ic, const double *prm, int64_t np, std::vect
 }
 
 double mass(const orc_case &c, const std::vector<double> &U) {
+  if (c.tri_scalar)
+    return wasl::scalar_mass(c.tri, U, c.n);
   return c.dim == 2 ? wasl::euler_density_mass(c.tri, U) : wasl::scalar_mass(c.line, U, c.n);
 }
 
 double stable_dt(const orc_case &c, const std::vector<double> &U, double cfl) {
+  if (c.tri_scalar)
+    return wasl::triangle_scalar_cfl_timestep(c.tri, c.advection.velocity_x,
+                                              c.advection.velocity_y, c.p, cfl);
   return c.dim == 2 ? wasl::euler_cfl_timestep(c.tri, c.equation, U, c.n, c.p, cfl)
                     : wasl::advection_cfl_timestep(c.line, c.scalar, c.p, cfl);
 }
@@ -260,6 +294,10 @@ void step(orc_case &c, double dt, int limiting, double thr, double rf, double pf
   const wasl::ResidualFunction res = [&c](std::vector<double> &stage, std::vector<double> &rhs) {
     residual(c, stage, rhs);
   };
+  if (c.tri_scalar) {
+    wasl::ssp_rk3_step(dt, res, U, c.workspace);
+    return;
+  }
   const wasl::StateFunction post = [&](std::vector<double> &s) {
     if (limiting)
       wasl::limit_euler_modal(c.stencil, c.equation, thr, rf, pf, s);
@@ -298,6 +336,28 @@ orc_case *orc_tri_euler_create(int64_t node_count, const int64_t *node_tags, con
           c->trace.point_count = order + 1;
         }
         c->stencil = wasl::make_euler_limiter_stencil(c->tri, order, donors, order + 1);
+      }) != 0) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+orc_case *orc_tri_scalar_create(int64_t node_count, const int64_t *node_tags, const double *node_xy,
+                                int64_t triangle_count, const int64_t *triangle_nodes, int order,
+                                double velocity_x, double velocity_y) {
+  auto *c = new orc_case;
+  if (guarded([&] {
+        c->dim = 2;
+        c->tri_scalar = true;
+        c->p = order;
+        c->n = wasl::triangle_mode_count(order);
+        c->boundary = ORC_PERIODIC;
+        c->advection = wasl::ScalarAdvection2D{velocity_x, velocity_y};
+        const auto built = wasl::build_triangle_mesh(
+            triangle_document(node_count, node_tags, node_xy, triangle_count, triangle_nodes));
+        c->tri = wasl::triangle_mesh_from_prepared(wasl::prepare_mesh(built));
+        c->trace = wasl::periodic_triangle_trace(c->tri, order);
       }) != 0) {
     delete c;
     return nullptr;
@@ -403,7 +463,7 @@ int orc_tri_trace(const orc_case *c, int32_t *donors, double *basis) {
 }
 
 int64_t orc_stencil_check_count(const orc_case *c) {
-  return c->dim == 2 ? static_cast<int64_t>(c->stencil.check_point_count()) : 0;
+  return c->dim == 2 && !c->tri_scalar ? static_cast<int64_t>(c->stencil.check_point_count()) : 0;
 }
 
 int orc_stencil(const orc_case *c, int32_t *neighbors, double *transforms, double *normals,
@@ -432,7 +492,7 @@ int orc_residual(const orc_case *c, const double *state, double *rhs) {
 int orc_limit(const orc_case *c, double threshold, double density_floor, double pressure_floor,
               double *state, uint8_t *flags) {
   return guarded([&] {
-    if (c->dim != 2)
+    if (c->dim != 2 || c->tri_scalar)
       throw std::runtime_error("oracle: the limiter is implemented for 2D Euler");
     std::vector<double> U(state, state + 4 * c->n * c->E());
     if (flags)

@@ -26,6 +26,13 @@ JUMP = (0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.2, 1.0,
         2, 0, 0, 0.1, -0.2, 0.45, 0.2, 0.0, 0.0, 0.15)
 
 
+# 2D scalar advection: plane wave sin(2 pi (kx x + ky y) / L) on [0,10]^2 and
+# a disc of value 2 on a 0.25 background (circles: default, cx, cy, r, value)
+SCALAR_WAVE = (10.0, 1.0, 2.0)
+SCALAR_DISC = (0.25, 5.0, 4.0, 2.5, 2.0)
+ADVECTION = (0.7, -0.4)
+
+
 def rel_inf(a, b) -> float:
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
@@ -67,6 +74,10 @@ def odd_mesh(n=9, seed=MG.DEFAULT_SEED):
 
 def oracle_tri(m, order, boundary=O.PERIODIC, kind="port"):
     return O.tri_euler(m.node_tags, m.node_xy, m.triangles, order, boundary=boundary, kind=kind)
+
+
+def oracle_tri_scalar(m, order, kind="port"):
+    return O.tri_scalar(m.node_tags, m.node_xy, m.triangles, order, *ADVECTION, kind=kind)
 
 
 def engine_tri(wasl, m, order, boundary=0, limiting=False, threshold=1e-3, floors=1e-10,

@@ -1,7 +1,8 @@
 """Writes the golden fixtures under tests/golden/ (run in a container where
 the reference is present: `python tests/golden/make_fixtures.py`).
 
-  case_regression_rows.txt  the 14 on-path rows (euler-d2-*, scalar-d1-*) of
+  case_regression_rows.txt  the 22 on-path rows (euler-d2-*, scalar-d1-*,
+                            scalar-d2-*) of
                             the reference's stored case-regression metrics
                             (cpp/tests/runtime/case_regression_reference.txt),
                             copied verbatim as data;
@@ -9,7 +10,8 @@ the reference is present: `python tests/golden/make_fixtures.py`).
                             (oracle/_ref, compiled from the reference sources)
                             on small seeded meshes: residuals P0-P3, limiter
                             flags and states, stepping trajectories, 1D
-                            advection.
+                            advection, 2D scalar advection residuals and
+                            steps.
 
 The reference cannot travel to the GPU box; these vectors do.
 """
@@ -33,8 +35,7 @@ def regression_rows() -> None:
     keep = []
     with open(REFERENCE_TXT) as fh:
         for line in fh:
-            if line.startswith("#") or line.startswith("euler-d2-") or line.startswith(
-                    "scalar-d1-"):
+            if line.startswith(("#", "euler-d2-", "scalar-d1-", "scalar-d2-")):
                 keep.append(line)
     with open(os.path.join(HERE, "case_regression_rows.txt"), "w") as fh:
         fh.writelines(keep)
@@ -91,6 +92,23 @@ def vectors() -> None:
         traj.append(U)
     out["pulse_p2_steps"] = np.stack(traj)
     out["pulse_p2_dt"] = np.array([dt])
+
+    for p in range(4):
+        c = O.tri_scalar(m.node_tags, m.node_xy, m.triangles, p, 0.7, -0.4, kind="reference")
+        U = c.project(O.IC_SCALAR_WAVE, K.SCALAR_WAVE)
+        rng = np.random.default_rng(200 + p)
+        U = U + 1e-3 * rng.standard_normal(U.size)
+        out[f"advect2d_p{p}_state"] = U
+        out[f"advect2d_p{p}_residual"] = c.residual(U)
+        out[f"advect2d_p{p}_dt"] = np.array([c.stable_dt(U, 0.3)])
+    c = O.tri_scalar(m.node_tags, m.node_xy, m.triangles, 2, 0.7, -0.4, kind="reference")
+    U = c.project(O.IC_SCALAR_CIRCLES, K.SCALAR_DISC)
+    dt = c.stable_dt(U, 0.3)
+    traj = [U]
+    for _ in range(5):
+        U = c.step(U, dt)
+        traj.append(U)
+    out["advect2d_p2_steps"] = np.stack(traj)
     np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
 
 

@@ -48,6 +48,18 @@ def test_coarse_calls_reproduce_case_regression_rows(gpu, tmp_path):
         s = case.run_scalar_advection_case(line_path, tmp_path / (name + "-restart"), ck,
                                            end_time=0.3, **args)
         got[name + "-restart"], _ = _measure(case, s)
+    for p in range(4):
+        name = f"scalar-d2-p{p}"
+        args = dict(mpi=False, dimensions=2, polynomial_order=p, velocity_x=0.7, velocity_y=-0.4,
+                    cfl=0.3, start_time=0.0, max_steps=1000, history_interval=3,
+                    visualization_interval=4, checkpoint_interval=5, keep_checkpoints=2,
+                    initial_default=0.25, initial_sine=False, initial_sine_amplitude=1.5,
+                    intervals=[0.2, 0.55, 1.0], circles=[0.5, 0.4, 0.25, 2.0])
+        s = case.run_scalar_advection_case(box_path, tmp_path / name, "", end_time=0.2, **args)
+        got[name], ck = _measure(case, s)
+        s = case.run_scalar_advection_case(box_path, tmp_path / (name + "-restart"), ck,
+                                           end_time=0.3, **args)
+        got[name + "-restart"], _ = _measure(case, s)
     for p in range(3):
         for lim in (False, True):
             name = f"euler-d2-p{p}" + ("-limited" if lim else "")

@@ -1,5 +1,5 @@
 """End-to-end runs through the HIP engine against the reference's own stored
-results: the 14 on-path case-regression rows (relative 1e-10, the reference's
+results: the 22 on-path case-regression rows (relative 1e-10, the reference's
 tolerance, case_regression_test.cpp:40-44), vectors written by the reference's
 code (tests/golden/reference_vectors.npz) and the vortex convergence order."""
 
@@ -25,6 +25,10 @@ class _GpuCase:
         if kind == "line":
             self.plain = wasl.Engine(wasl.Mesh.lines(*mesh), wasl.SCALAR_ADVECTION, p,
                                      wavespeed=0.7)
+            self.limited = None
+        elif kind == "tri_scalar":
+            self.plain = wasl.Engine(wasl.Mesh.triangles(*mesh), wasl.SCALAR_ADVECTION, p,
+                                     wavespeed=0.7, velocity_y=-0.4)
             self.limited = None
         else:
             m = wasl.Mesh.triangles(*mesh)

@@ -311,6 +311,73 @@ def test_free_stream_and_conservation(p):
         assert abs(comp.sum()) <= 1e-12 * max(1.0, np.abs(comp).sum())
 
 
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_scalar_2d_constant_and_conservation(p):
+    """A constant is a steady state of the 2D scalar residual, and the residual
+    conserves the integral: sum_e sqrt(2) detJ_e rhs[e][0] = 0."""
+    m = K.vortex_mesh(6)
+    c = K.oracle_tri_scalar(m, p)
+    U = c.project(O.IC_SCALAR_CIRCLES, (1.7,))
+    assert np.all(np.abs(U.reshape(c.E, c.Np)[:, 1:]) <= 1e-14)
+    assert np.abs(c.residual(U)).max() <= 1e-12
+    V = c.project(O.IC_SCALAR_CIRCLES, K.SCALAR_DISC)
+    Rr = c.residual(V).reshape(c.E, c.Np)
+    detj = np.array([((t[2] - t[0]) * (t[5] - t[1]) - (t[4] - t[0]) * (t[3] - t[1])) / 4
+                     for t in c.topology()["element_xy"]])
+    comp = math.sqrt(2) * detj * Rr[:, 0]
+    assert abs(comp.sum()) <= 1e-12 * np.abs(comp).sum()
+    # the step keeps the mass to round-off
+    dt = c.stable_dt(V, 0.3)
+    assert abs(c.mass(c.step(V, dt)) - c.mass(V)) <= 1e-13 * abs(c.mass(V))
+
+
+def test_scalar_2d_upwind_direction():
+    """Flip the velocity: the residual of the mirrored problem is the residual
+    with the upwind side swapped. With v = 0 the residual vanishes; the CFL
+    step follows cfl h_min / (|v| (2p+1)) with h_min the smallest altitude."""
+    m = K.vortex_mesh(5)
+    c0 = O.tri_scalar(m.node_tags, m.node_xy, m.triangles, 1, 0.0, 0.0)
+    U = c0.project(O.IC_SCALAR_WAVE, K.SCALAR_WAVE)
+    assert np.abs(c0.residual(U)).max() == 0.0
+    assert c0.stable_dt(U, 0.3) == 0.0
+    c = O.tri_scalar(m.node_tags, m.node_xy, m.triangles, 1, 3.0, 4.0)
+    xy = c.topology()["element_xy"]
+    hmin = np.inf
+    for t in xy:
+        det = ((t[2] - t[0]) * (t[5] - t[1]) - (t[4] - t[0]) * (t[3] - t[1])) / 4
+        edges = [math.hypot(t[2] - t[0], t[3] - t[1]), math.hypot(t[4] - t[2], t[5] - t[3]),
+                 math.hypot(t[0] - t[4], t[1] - t[5])]
+        hmin = min(hmin, 4 * det / max(edges))
+    assert abs(c.stable_dt(U, 0.3) - 0.3 * hmin / (5.0 * 3.0)) <= 1e-15
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_scalar_2d_plane_wave_convergence(p):
+    """Advected plane wave u(x - v t) on jittered periodic meshes: the L2 error
+    falls at rate >= p + 0.5 (the reference asks p + 0.7 on structured unit
+    squares, triangle_residual_test.cpp:252-283)."""
+    Rt = O.Routines()
+    xi, eta, w = Rt.rule(1, 6)
+    B = np.array([Rt.dubiner(a, b, p) for a, b in zip(xi, eta)])
+    vx, vy = K.ADVECTION
+    errs = []
+    for n in (6, 12):
+        m = K.vortex_mesh(n)
+        c = K.oracle_tri_scalar(m, p)
+        U, _, t, _ = c.run(c.project(O.IC_SCALAR_WAVE, K.SCALAR_WAVE), 0.3, 1.0)
+        U = U.reshape(c.E, c.Np)
+        e2 = 0.0
+        for e, (x0, y0, x1, y1, x2, y2) in enumerate(c.topology()["element_xy"]):
+            det = ((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0)) / 4
+            X = x0 + 0.5 * ((x1 - x0) * (xi + 1) + (x2 - x0) * (eta + 1))
+            Y = y0 + 0.5 * ((y1 - y0) * (xi + 1) + (y2 - y0) * (eta + 1))
+            L, kx, ky = K.SCALAR_WAVE
+            exact = np.sin(2 * math.pi * (kx * (X - vx * t) + ky * (Y - vy * t)) / L)
+            e2 += np.sum(w * det * (B @ U[e] - exact) ** 2)
+        errs.append(math.sqrt(e2))
+    assert math.log2(errs[0] / errs[1]) >= p + 0.5, errs
+
+
 def test_rk3_temporal_order():
     """Two P0 cells, periodic, a = 1: the semi-discrete solution decays the
     difference as exp(-4t) (ssp_rk3_test.cpp:155-191); rate >= 2.5."""
@@ -399,6 +466,13 @@ def run_regression(kind_factory):
         out[f"scalar-d1-p{p}-restart"] = (st, t, chk.sum(), math.sqrt((chk * chk).sum()),
                                           h[-1, 3])
     bt, bxy, btri = RM.irregular_box(5)
+    for p in range(4):
+        c = kind_factory("tri_scalar", (bt, bxy, btri), p)
+        prm = [0.25, 0.5, 0.4, 0.25, 2.0]
+        U, st, t, h = c.run(c.project(O.IC_SCALAR_CIRCLES, prm), 0.3, 0.2, 1000, 3)
+        out[f"scalar-d2-p{p}"] = (st, t, U.sum(), math.sqrt((U * U).sum()), h[-1, 3])
+        U, st, t, h = c.run(U, 0.3, 0.3, 1000, 3, start_time=t, start_step=st)
+        out[f"scalar-d2-p{p}-restart"] = (st, t, U.sum(), math.sqrt((U * U).sum()), h[-1, 3])
     for p in range(3):
         for lim in (False, True):
             c = kind_factory("tri", (bt, bxy, btri), p)
@@ -414,11 +488,13 @@ def run_regression(kind_factory):
 
 
 def test_case_regression_rows():
-    """The 14 on-path rows of the reference's stored case-regression metrics
+    """The 22 on-path rows of the reference's stored case-regression metrics
     (cpp/tests/runtime/case_regression_reference.txt) at relative 1e-10."""
     def factory(kind, mesh, p):
         if kind == "line":
             return O.line_scalar(*mesh, p, 0.7)
+        if kind == "tri_scalar":
+            return O.tri_scalar(*mesh, p, 0.7, -0.4)
         return O.tri_euler(*mesh, p)
 
     got = run_regression(factory)
@@ -469,6 +545,19 @@ def test_port_reproduces_reference_vectors():
     for k in range(1, traj.shape[0]):
         U = c.step(U, f["pulse_p2_dt"][0])
         assert np.array_equal(U, traj[k])
+    for p in range(4):
+        c = K.oracle_tri_scalar(m, p)
+        U = f[f"advect2d_p{p}_state"]
+        assert np.array_equal(c.residual(U), f[f"advect2d_p{p}_residual"])
+        assert c.stable_dt(U, 0.3) == f[f"advect2d_p{p}_dt"][0]
+    c = K.oracle_tri_scalar(m, 2)
+    traj = f["advect2d_p2_steps"]
+    assert np.array_equal(c.project(O.IC_SCALAR_CIRCLES, K.SCALAR_DISC), traj[0])
+    dt = c.stable_dt(traj[0], 0.3)
+    U = traj[0]
+    for k in range(1, traj.shape[0]):
+        U = c.step(U, dt)
+        assert np.array_equal(U, traj[k])
 
 
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
@@ -494,6 +583,26 @@ def test_port_bitwise_equals_reference_build(reference_oracle, p):
         assert dt == b.stable_dt(la, 0.1)
         assert np.array_equal(a.step(la, dt, True, 1e-3, 1e-13, 1e-13),
                               b.step(la, dt, True, 1e-3, 1e-13, 1e-13))
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_port_scalar_2d_bitwise_equals_reference_build(reference_oracle, p):
+    """2D scalar advection (triangle_residual.hpp, scalar_advection_triangle.cpp)
+    against the reference build, bit for bit, on a jittered periodic mesh."""
+    m = K.vortex_mesh(9)
+    a = K.oracle_tri_scalar(m, p)
+    b = K.oracle_tri_scalar(m, p, kind="reference")
+    assert all(np.array_equal(x, y) for x, y in zip(a.trace(), b.trace()))
+    for ic, prm in ((O.IC_SCALAR_WAVE, K.SCALAR_WAVE), (O.IC_SCALAR_CIRCLES, K.SCALAR_DISC)):
+        U = a.project(ic, prm)
+        assert np.array_equal(U, b.project(ic, prm))
+        assert np.array_equal(a.residual(U), b.residual(U))
+        assert a.mass(U) == b.mass(U)
+        dt = a.stable_dt(U, 0.3)
+        assert dt == b.stable_dt(U, 0.3)
+        assert np.array_equal(a.step(U, dt), b.step(U, dt))
+    with pytest.raises(O.OracleError):
+        a.limit(U, 1e-3)
 
 
 def test_routines_bitwise_equal_reference_build(reference_oracle):

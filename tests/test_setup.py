@@ -43,6 +43,22 @@ def test_setup_bitwise_equals_oracle(wasl, order, boundary):
     assert np.array_equal(mesh.project(order, ic, prm), o.project(ic, prm))
 
 
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_scalar_2d_projection_bitwise_equals_oracle(wasl, order):
+    """2D scalar initial states (circles, plane wave) in U[element][mode]."""
+    m = K.vortex_mesh(7)
+    mesh = wasl.Mesh.from_input(m)
+    o = K.oracle_tri_scalar(m, order)
+    for ic, prm in ((O.IC_SCALAR_CIRCLES, K.SCALAR_DISC), (O.IC_SCALAR_WAVE, K.SCALAR_WAVE)):
+        assert np.array_equal(mesh.project(order, ic, prm), o.project(ic, prm))
+    bt, bxy, btri = RM.irregular_box(5)
+    box = wasl.Mesh.triangles(bt, bxy, btri)
+    ob = O.tri_scalar(bt, bxy, btri, order, 0.7, -0.4)
+    prm = [0.25, 0.5, 0.4, 0.25, 2.0]
+    assert np.array_equal(box.project(order, O.IC_SCALAR_CIRCLES, prm),
+                          ob.project(O.IC_SCALAR_CIRCLES, prm))
+
+
 def test_regression_meshes_match(wasl):
     """The reference test meshes with shuffled records and orientations."""
     bt, bxy, btri = RM.irregular_box(5)
@@ -144,3 +160,5 @@ def test_numerics_defaults(wasl):
     wasl.library().wasl_hip_numerics_default(ctypes.byref(n))
     assert n.sensor_threshold == 1e-3 and n.density_floor == 1e-10
     assert n.pressure_floor == 1e-10 and n.heat_capacity_ratio == 1.4
+    assert n.velocity_y == 0.0
+    assert ctypes.sizeof(n) == 104   # sizeof(wasl_hip_numerics) on x86-64

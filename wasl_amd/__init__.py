@@ -9,7 +9,7 @@ interfaces plus the synthetic mesh generator.
 
 from . import meshgen, serve  # noqa: F401
 from .engine import (  # noqa: F401
-    BF16, EULER, FP64, IC_EULER_REGIONS, IC_RAREFACTION, IC_SCALAR_PULSE, IC_SCALAR_REGIONS,
-    IC_SMOOTH, IC_VORTEX, INFLOW, MIXED, OUTFLOW, PERIODIC, SCALAR_ADVECTION, Engine, Mesh,
-    SimulationError, WaslHipError, library,
+    BF16, EULER, FP64, IC_EULER_REGIONS, IC_RAREFACTION, IC_SCALAR_CIRCLES, IC_SCALAR_PULSE,
+    IC_SCALAR_REGIONS, IC_SCALAR_WAVE, IC_SMOOTH, IC_VORTEX, INFLOW, MIXED, OUTFLOW, PERIODIC,
+    SCALAR_ADVECTION, Engine, Mesh, SimulationError, WaslHipError, library,
 )

@@ -426,27 +426,34 @@ SimulationSummary run_euler_case(const EulerCase &config) {
 }
 
 // src/case/scalar_advection_case.cpp:8-16 with run_line_scalar_advection
-// (src/case/scalar_advection_line.cpp:12-34).
+// (src/case/scalar_advection_line.cpp:12-34) and run_triangle_scalar_advection
+// (src/case/scalar_advection_triangle.cpp:15-46).
 SimulationSummary run_scalar_advection_case(const ScalarAdvectionCase &config) {
   validate_scalar_advection_case(config);
   const auto mesh = load_case_mesh(config.files.prepared_mesh, config.dimensions);
-  if (config.dimensions == 2)
-    throw std::runtime_error(
-        "wasl_hip: scalar advection on triangles is not part of this engine");
   if (config.mpi)
-    throw std::runtime_error("wasl_hip: MPI runs are not part of this engine (one case per GPU)");
-  BoundaryCondition left, right;
-  resolve_line_boundaries(mesh.prepared, config.boundaries, left, right);
-  auto m = engine_mesh(mesh.prepared);
+    throw std::runtime_error(config.dimensions == 2
+                                 ? "simulation: 2D case-level MPI is not implemented"
+                                 : "wasl_hip: MPI runs are not part of this engine (one case per GPU)");
   wasl_hip_numerics num;
   wasl_hip_numerics_default(&num);
   num.equation = WASL_HIP_SCALAR_ADVECTION;
   num.polynomial_order = config.polynomial_order;
   num.wavespeed = config.wavespeed;
-  num.boundary_left = static_cast<int>(left.kind);
-  num.boundary_right = static_cast<int>(right.kind);
-  num.boundary_value_left = left.value;
-  num.boundary_value_right = right.value;
+  if (config.dimensions == 2) {
+    if (!all_of_kind(config.boundaries, BoundaryKind::periodic))
+      throw std::runtime_error("simulation: 2D scalar-advection boundaries must be periodic");
+    num.velocity_y = config.velocity_y;
+    num.boundary_left = num.boundary_right = WASL_HIP_PERIODIC;
+  } else {
+    BoundaryCondition left, right;
+    resolve_line_boundaries(mesh.prepared, config.boundaries, left, right);
+    num.boundary_left = static_cast<int>(left.kind);
+    num.boundary_right = static_cast<int>(right.kind);
+    num.boundary_value_left = left.value;
+    num.boundary_value_right = right.value;
+  }
+  auto m = engine_mesh(mesh.prepared);
   DeviceCase dev;
   wasl_hip_engine *g = nullptr;
   call(wasl_hip_engine_create(m.get(), &num, &g));
@@ -460,11 +467,20 @@ SimulationSummary run_scalar_advection_case(const ScalarAdvectionCase &config) {
   SerialCaseModel model;
   dev.bind(model);
   model.initial_state = [&] {
+    std::vector<double> U(static_cast<std::size_t>(dev.Np) * dev.E);
+    if (config.dimensions == 2) {
+      // the first containing circle, else the default (initial_regions.cpp:28-34)
+      std::vector<double> packed{config.initial_default};
+      for (const auto &c : config.initial_circles)
+        packed.insert(packed.end(), {c.center_x, c.center_y, c.radius, c.value});
+      call(wasl_hip_project(dev.engine.get(), 12, packed.data(),
+                            static_cast<int64_t>(packed.size()), U.data()));
+      return;
+    }
     std::vector<double> packed{config.initial_default, config.initial_sine ? 1.0 : 0.0,
                                config.initial_sine_amplitude};
     for (const auto &r : config.initial_intervals)
       packed.insert(packed.end(), {r.lower, r.upper, r.value});
-    std::vector<double> U(static_cast<std::size_t>(dev.Np) * dev.E);
     call(wasl_hip_project(dev.engine.get(), 11, packed.data(), static_cast<int64_t>(packed.size()),
                           U.data()));
   };

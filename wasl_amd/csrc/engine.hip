@@ -166,7 +166,11 @@ struct wasl_hip_engine {
 
   size_t state_size() const { return static_cast<size_t>(V) * Np * E; }
 
-  bool mixed() const { return dim == 2 && num.precision != WASL_HIP_FP64; }
+  // 2D Euler on triangles; otherwise scalar advection on lines (dim 1) or
+  // periodic triangles (dim 2).
+  bool euler() const { return num.equation == WASL_HIP_EULER; }
+
+  bool mixed() const { return euler() && num.precision != WASL_HIP_FP64; }
 
   // The trace buffer that belongs to a state buffer (mixed precision only).
   float *traces_of(const double *p) const {
@@ -193,6 +197,19 @@ struct wasl_hip_engine {
     m.table = table.as<double>();
     m.vtable = vtable.as<double>();
     m.gamma = num.heat_capacity_ratio;
+    return m;
+  }
+
+  whip::AdvectMesh advect_mesh() const {
+    whip::AdvectMesh m;
+    m.E = E;
+    m.vel = geo.as<double>();
+    m.nbr = nbr.as<int>();
+    m.meta = meta.as<int>();
+    m.flux = fn.as<double>();
+    m.bdonor = bdonor.as<int>();
+    m.bbasis = bbasis.as<double>();
+    m.table = table.as<double>();
     return m;
   }
 
@@ -223,7 +240,7 @@ struct wasl_hip_engine {
   }
 
   void build_stencil_host() {
-    if (have_stencil || dim != 2)
+    if (have_stencil || !euler())
       return;
     stencil = whip::limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC
                                                  ? trace.donors
@@ -236,6 +253,27 @@ struct wasl_hip_engine {
 namespace {
 
 // ------------------------------------------------------------- tables ----
+
+// Edge tables shared by the triangle stages: own-side basis at the edge
+// points bl [3][NE][Np], the neighbour's basis at the matching points
+// br [kl][kr][flip][NE][Np], edge weights we [3][NE].
+void edge_tables(int p, double *bl, double *br, double *we) {
+  const int Np = whip::tri_modes(p), NE = p + 1;
+  const auto edges = whip::edge_rules(p);
+  for (int k = 0; k < 3; ++k)
+    for (int q = 0; q < NE; ++q) {
+      whip::dubiner(edges[k].xi[q], edges[k].eta[q], p, bl + (k * NE + q) * Np);
+      we[k * NE + q] = edges[k].w[q];
+    }
+  for (int kl = 0; kl < 3; ++kl)
+    for (int kr = 0; kr < 3; ++kr)
+      for (int fl = 0; fl < 2; ++fl)
+        for (int q = 0; q < NE; ++q) {
+          double rxi, reta;
+          whip::neighbor_point(kl, kr, fl != 0, edges[kl].xi[q], edges[kl].eta[q], rxi, reta);
+          whip::dubiner(rxi, reta, p, br + (((kl * 3 + kr) * 2 + fl) * NE + q) * Np);
+        }
+}
 
 std::vector<double> euler_tables(int p) {
   const int Np = whip::tri_modes(p), NQ = (p + 3) * (p + 3), NE = p + 1;
@@ -253,20 +291,30 @@ std::vector<double> euler_tables(int p) {
       t[kWy + q * Np + m] = vol.w[q] * ge[m];
     }
   }
-  const auto edges = whip::edge_rules(p);
-  for (int k = 0; k < 3; ++k)
-    for (int q = 0; q < NE; ++q) {
-      whip::dubiner(edges[k].xi[q], edges[k].eta[q], p, t.data() + kBl + (k * NE + q) * Np);
-      t[kWe + k * NE + q] = edges[k].w[q];
-    }
-  for (int kl = 0; kl < 3; ++kl)
-    for (int kr = 0; kr < 3; ++kr)
-      for (int fl = 0; fl < 2; ++fl)
-        for (int q = 0; q < NE; ++q) {
-          double rxi, reta;
-          whip::neighbor_point(kl, kr, fl != 0, edges[kl].xi[q], edges[kl].eta[q], rxi, reta);
-          whip::dubiner(rxi, reta, p, t.data() + kBr + (((kl * 3 + kr) * 2 + fl) * NE + q) * Np);
-        }
+  edge_tables(p, t.data() + kBl, t.data() + kBr, t.data() + kWe);
+  return t;
+}
+
+// 2D scalar advection (whip::TriAdv<P>): the volume term as differentiation
+// matrices Sx[m][n] = sum_q w_q dphi_m/dxi phi_n, Sy likewise with d/deta, over
+// the order-p rule of the reference's residual (triangle_residual.hpp:31), then
+// the edge tables.
+std::vector<double> advect_tables(int p) {
+  const int Np = whip::tri_modes(p), NE = p + 1;
+  const int kSy = Np * Np, kBl = 2 * Np * Np, kBr = kBl + 3 * NE * Np, kWe = kBr + 18 * NE * Np;
+  std::vector<double> t(kWe + 3 * NE, 0.0);
+  const whip::TriRule vol = whip::triangle_rule(p);
+  std::vector<double> b(Np), gx(Np), ge(Np);
+  for (int q = 0; q < vol.size(); ++q) {
+    whip::dubiner(vol.xi[q], vol.eta[q], p, b.data());
+    whip::dubiner_gradient(vol.xi[q], vol.eta[q], p, gx.data(), ge.data());
+    for (int m = 0; m < Np; ++m)
+      for (int n = 0; n < Np; ++n) {
+        t[m * Np + n] += vol.w[q] * gx[m] * b[n];
+        t[kSy + m * Np + n] += vol.w[q] * ge[m] * b[n];
+      }
+  }
+  edge_tables(p, t.data() + kBl, t.data() + kBr, t.data() + kWe);
   return t;
 }
 
@@ -424,13 +472,25 @@ void launch_euler_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
   }
 }
 
+template <int P> void launch_advect_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
+  const dim3 grid((g.E + 255) / 256);
+  hipLaunchKernelGGL((whip::advect_stage<P>), grid, dim3(256), 0, g.stream, g.advect_mesh(), a);
+}
+
 template <int P> void launch_line_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
   const dim3 grid((g.E + 255) / 256);
   hipLaunchKernelGGL((whip::line_stage<P>), grid, dim3(256), 0, g.stream, g.line_mesh(), a);
 }
 
 void launch_stage(const wasl_hip_engine &g, const whip::StageArgs &a) {
-  if (g.dim == 2) {
+  if (g.dim == 2 && !g.euler()) {
+    switch (g.p) {
+    case 0: launch_advect_p<0>(g, a); break;
+    case 1: launch_advect_p<1>(g, a); break;
+    case 2: launch_advect_p<2>(g, a); break;
+    default: launch_advect_p<3>(g, a); break;
+    }
+  } else if (g.dim == 2) {
     switch (g.p) {
     case 0: launch_euler_p<0>(g, a); break;
     case 1: launch_euler_p<1>(g, a); break;
@@ -499,7 +559,7 @@ template <int P> void launch_sensor_p(const wasl_hip_engine &g, uint8_t *flags) 
                      flags, g.list.as<int>(), g.counters.as<int>() + 3);
 }
 
-bool limiting(const wasl_hip_engine &g) { return g.dim == 2 && g.num.limiting && g.p > 0; }
+bool limiting(const wasl_hip_engine &g) { return g.euler() && g.num.limiting && g.p > 0; }
 
 // Shu-Osher stages (time/ssp_rk3.hpp:45-60): U1 = U + dt R(U);
 // U2 = 3/4 U + 1/4 (U1 + dt R(U1)); U = 1/3 U + 2/3 (U2 + dt R(U2)); the
@@ -524,7 +584,7 @@ void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, b
     a.parity = parity;
     a.stage = s;
     a.residual_only = 0;
-    a.want_lambda = (g.dim == 2 && ctl && s == 2) ? 1 : 0;
+    a.want_lambda = (g.euler() && ctl && s == 2) ? 1 : 0;
     a.sensor = lim ? 1 : 0;
     a.threshold = g.num.sensor_threshold;
     a.highest_begin = g.stencil.highest_begin;
@@ -578,7 +638,7 @@ void raise_device_error(wasl_hip_engine &g) {
 double mass_of(wasl_hip_engine &g) {
   const int blocks = (g.E + 255) / 256;
   hipLaunchKernelGGL(whip::mass_partials, dim3(blocks), dim3(256), 0, g.stream,
-                     g.u.as<double>(), g.dim == 2 ? 1 : g.Np, g.mass_w.as<double>(), g.E,
+                     g.u.as<double>(), g.euler() ? 1 : g.Np, g.mass_w.as<double>(), g.E,
                      g.partials.as<double>());
   hipLaunchKernelGGL(whip::mass_finish, dim3(1), dim3(64), 0, g.stream, g.partials.as<double>(),
                      blocks, g.result.as<double>());
@@ -590,10 +650,14 @@ double mass_of(wasl_hip_engine &g) {
 }
 
 // Largest |u|+c of the resident state's cell averages (throws when one is
-// unphysical), or |a| for advection.
+// unphysical), or the advection speed |a| (1D) or |v| (2D).
+double advection_speed(const wasl_hip_engine &g) {
+  return g.dim == 1 ? std::fabs(g.num.wavespeed) : std::hypot(g.num.wavespeed, g.num.velocity_y);
+}
+
 double resident_speed(wasl_hip_engine &g) {
-  if (g.dim == 1)
-    return std::fabs(g.num.wavespeed);
+  if (!g.euler())
+    return advection_speed(g);
   whip::RunCtl *h = g.ctl_host;
   std::memset(h, 0, sizeof(*h));
   HIPCHECK(hipMemcpyAsync(g.ctl.p, h, sizeof(*h), hipMemcpyHostToDevice, g.stream));
@@ -694,6 +758,78 @@ void autotune(wasl_hip_engine &g) {
   g_tuned.emplace(key, best);
 }
 
+// Edge records of a triangle mesh, gathered per element so a stage kernel
+// reads its three faces without atomics: neighbour element (-1 across the
+// boundary) and meta bits per [edge][element], the face normal (from the left
+// side) and edge scale per [3][edge][element], and for each boundary face (in
+// face order, its slot in the meta bits) the periodic donors and donor basis
+// of its edge points when a trace is given.
+struct EdgeRecords {
+  std::vector<int> nbr, meta, bdon;
+  std::vector<double> fn, bbas;
+  int slots = 0;
+};
+
+EdgeRecords edge_records(const whip::Mesh &mesh, int p, int Np, const whip::PeriodicTrace *trace) {
+  const size_t E = mesh.tris.size();
+  const int NE = p + 1;
+  EdgeRecords r;
+  r.nbr.assign(3 * E, -2);
+  r.meta.assign(3 * E, 0);
+  r.fn.assign(9 * E, 0.0);
+  const auto edges = whip::edge_rules(p);
+  int slot = 0;
+  for (size_t f = 0; f < mesh.faces.size(); ++f) {
+    const whip::TriFace &x = mesh.faces[f];
+    const whip::Triangle &l = mesh.tris[x.left];
+    const double scale = l.edge_length(x.left_edge) / whip::reference_edge_length(x.left_edge);
+    const int flip = x.boundary() ? 0 : (x.flip() ? 1 : 0);
+    const int kr = x.boundary() ? 0 : x.right_edge;
+    int m = (x.left_edge << 1) | (kr << 3) | (flip << 5);
+    const auto put = [&](int e, int k, int other, int bits) {
+      const size_t i = static_cast<size_t>(k) * E + e;
+      if (r.nbr[i] != -2)
+        throw std::runtime_error("engine: element edge belongs to two faces");
+      r.nbr[i] = other;
+      r.meta[i] = bits;
+      r.fn[(3 * k) * E + e] = x.nx;
+      r.fn[(3 * k + 1) * E + e] = x.ny;
+      r.fn[(3 * k + 2) * E + e] = scale;
+    };
+    if (x.boundary()) {
+      put(x.left, x.left_edge, -1, m | whip::kMetaLeft | whip::kMetaBoundary | (slot << 8));
+      if (trace) {
+        for (int q = 0; q < NE; ++q) {
+          r.bdon.push_back(trace->donors[f * NE + q]);
+          for (int k = 0; k < Np; ++k)
+            r.bbas.push_back(trace->basis[(f * NE + q) * Np + k]);
+        }
+      }
+      ++slot;
+    } else {
+      // Whether the right side lists the face's Gauss points in the reverse
+      // order of the left side (each in its own edge rule), by position.
+      int rev = 0;
+      if (NE > 1) {
+        const whip::Triangle &rt = mesh.tris[x.right];
+        double lx, ly, r0x, r0y, r1x, r1y;
+        l.to_physical(edges[x.left_edge].xi[0], edges[x.left_edge].eta[0], lx, ly);
+        rt.to_physical(edges[x.right_edge].xi[0], edges[x.right_edge].eta[0], r0x, r0y);
+        rt.to_physical(edges[x.right_edge].xi[NE - 1], edges[x.right_edge].eta[NE - 1], r1x,
+                       r1y);
+        rev = std::hypot(lx - r0x, ly - r0y) > std::hypot(lx - r1x, ly - r1y) ? 1 : 0;
+      }
+      const int rb = rev ? whip::kMetaReverse : 0;
+      put(x.left, x.left_edge, x.right, m | whip::kMetaLeft | rb);
+      put(x.right, x.right_edge, x.left, m | rb);
+    }
+  }
+  if (std::find(r.nbr.begin(), r.nbr.end(), -2) != r.nbr.end())
+    throw std::runtime_error("engine: element edge without a face");
+  r.slots = slot;
+  return r;
+}
+
 void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hip_engine &g) {
   g.num = num;
   g.p = num.polynomial_order;
@@ -731,7 +867,6 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
       throw std::runtime_error("engine: the variable-split kernel takes 4 or 8 lanes per element");
     if (g.stage_kernel != 1 && g.stage_kernel != 2)
       throw std::runtime_error("engine: stage_kernel must be 0, 1 or 2");
-    const int NE = g.p + 1;
     const size_t E = static_cast<size_t>(g.E);
 
     // element metrics
@@ -748,59 +883,11 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     // edge records
     if (num.boundary_left == WASL_HIP_PERIODIC)
       g.trace = whip::periodic_trace(mesh, g.p);
-    std::vector<int> nbr(3 * E, -2), meta(3 * E, 0);
-    std::vector<double> fn(9 * E, 0.0);
-    std::vector<int> bdon;
-    std::vector<double> bbas;
-    const auto edges = whip::edge_rules(g.p);
-    int slot = 0;
-    for (size_t f = 0; f < mesh.faces.size(); ++f) {
-      const whip::TriFace &x = mesh.faces[f];
-      const whip::Triangle &l = mesh.tris[x.left];
-      const double scale = l.edge_length(x.left_edge) / whip::reference_edge_length(x.left_edge);
-      const int flip = x.boundary() ? 0 : (x.flip() ? 1 : 0);
-      const int kr = x.boundary() ? 0 : x.right_edge;
-      int m = (x.left_edge << 1) | (kr << 3) | (flip << 5);
-      const auto put = [&](int e, int k, int other, int bits) {
-        const size_t i = static_cast<size_t>(k) * E + e;
-        if (nbr[i] != -2)
-          throw std::runtime_error("engine: element edge belongs to two faces");
-        nbr[i] = other;
-        meta[i] = bits;
-        fn[(3 * k) * E + e] = x.nx;
-        fn[(3 * k + 1) * E + e] = x.ny;
-        fn[(3 * k + 2) * E + e] = scale;
-      };
-      if (x.boundary()) {
-        put(x.left, x.left_edge, -1, m | whip::kMetaLeft | whip::kMetaBoundary | (slot << 8));
-        if (num.boundary_left == WASL_HIP_PERIODIC) {
-          for (int q = 0; q < NE; ++q) {
-            bdon.push_back(g.trace.donors[f * NE + q]);
-            for (int k = 0; k < g.Np; ++k)
-              bbas.push_back(g.trace.basis[(f * NE + q) * g.Np + k]);
-          }
-        }
-        ++slot;
-      } else {
-        // Whether the right side lists the face's Gauss points in the reverse
-        // order of the left side (each in its own edge rule), by position.
-        int rev = 0;
-        if (NE > 1) {
-          const whip::Triangle &r = mesh.tris[x.right];
-          double lx, ly, r0x, r0y, r1x, r1y;
-          l.to_physical(edges[x.left_edge].xi[0], edges[x.left_edge].eta[0], lx, ly);
-          r.to_physical(edges[x.right_edge].xi[0], edges[x.right_edge].eta[0], r0x, r0y);
-          r.to_physical(edges[x.right_edge].xi[NE - 1], edges[x.right_edge].eta[NE - 1], r1x, r1y);
-          rev = std::hypot(lx - r0x, ly - r0y) > std::hypot(lx - r1x, ly - r1y) ? 1 : 0;
-        }
-        const int rb = rev ? whip::kMetaReverse : 0;
-        put(x.left, x.left_edge, x.right, m | whip::kMetaLeft | rb);
-        put(x.right, x.right_edge, x.left, m | rb);
-      }
-    }
-    if (std::find(nbr.begin(), nbr.end(), -2) != nbr.end())
-      throw std::runtime_error("engine: element edge without a face");
-    g.NB = slot;
+    EdgeRecords rec = edge_records(mesh, g.p, g.Np,
+                                   num.boundary_left == WASL_HIP_PERIODIC ? &g.trace : nullptr);
+    std::vector<int> &nbr = rec.nbr, &meta = rec.meta, &bdon = rec.bdon;
+    std::vector<double> &fn = rec.fn, &bbas = rec.bbas;
+    g.NB = rec.slots;
     upload(g.geo, geo, tot, g.stream);
     upload(g.mass_w, mw, tot, g.stream);
     upload(g.nbr, nbr, tot, g.stream);
@@ -840,9 +927,53 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     }
     g.flags.alloc(E, tot);
     g.list.alloc(E * sizeof(int), tot);
+  } else if (num.equation == WASL_HIP_SCALAR_ADVECTION && mesh.dimension == 2) {
+    // src/case/scalar_advection_triangle.cpp:15-46: periodic box, constant
+    // velocity (wavespeed, velocity_y), no limiter.
+    if (num.boundary_left != WASL_HIP_PERIODIC)
+      throw std::runtime_error("simulation: 2D scalar-advection boundaries must be periodic");
+    if (num.limiting)
+      throw std::runtime_error("check: scalar limiting is not supported");
+    if (num.precision != WASL_HIP_FP64)
+      throw std::runtime_error("engine: 2D scalar advection runs in FP64");
+    g.dim = 2;
+    g.V = 1;
+    g.Np = whip::tri_modes(g.p);
+    g.E = static_cast<int>(mesh.tris.size());
+    g.F = static_cast<int>(mesh.faces.size());
+    g.lanes = 1;
+    g.stage_kernel = 1;
+    const size_t E = static_cast<size_t>(g.E);
+    g.trace = whip::periodic_trace(mesh, g.p);
+    const EdgeRecords rec = edge_records(mesh, g.p, g.Np, &g.trace);
+    g.NB = rec.slots;
+    const double vx = num.wavespeed, vy = num.velocity_y;
+    std::vector<double> vel(2 * E), flux(3 * E), mw(E);
+    for (size_t e = 0; e < E; ++e) {
+      const whip::Triangle &t = mesh.tris[e];
+      const double inv = 1.0 / t.jacobian_determinant();
+      const double A = 0.5 * t.by(), B = -0.5 * t.bx(), C = -0.5 * t.ay(), D = 0.5 * t.ax();
+      vel[e] = (A * vx + B * vy) * inv;
+      vel[E + e] = (C * vx + D * vy) * inv;
+      mw[e] = std::sqrt(2.0) * t.jacobian_determinant();
+      for (int k = 0; k < 3; ++k) {
+        const double nx = rec.fn[(3 * k) * E + e], ny = rec.fn[(3 * k + 1) * E + e];
+        flux[k * E + e] = (vx * nx + vy * ny) * rec.fn[(3 * k + 2) * E + e] * inv;
+      }
+    }
+    upload(g.geo, vel, tot, g.stream);
+    upload(g.fn, flux, tot, g.stream);
+    upload(g.mass_w, mw, tot, g.stream);
+    upload(g.nbr, rec.nbr, tot, g.stream);
+    upload(g.meta, rec.meta, tot, g.stream);
+    upload(g.bdonor, rec.bdon, tot, g.stream);
+    upload(g.bbasis, rec.bbas, tot, g.stream);
+    upload(g.table, advect_tables(g.p), tot, g.stream);
+    g.hmin = whip::minimum_length(mesh);
+    g.stencil.highest_begin = g.Np;
   } else if (num.equation == WASL_HIP_SCALAR_ADVECTION) {
     if (mesh.dimension != 1)
-      throw std::runtime_error("engine: scalar advection runs on line meshes");
+      throw std::runtime_error("engine: scalar advection runs on line or triangle meshes");
     if (num.limiting)
       throw std::runtime_error("check: scalar limiting is not supported");
     const int kinds[2] = {num.boundary_left, num.boundary_right};
@@ -889,7 +1020,7 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
   g.result.alloc(sizeof(double), tot);
   HIPCHECK(hipStreamSynchronize(g.stream));
-  if (g.dim == 2 && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
+  if (g.euler() && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
       num.lanes_per_element == 0)
     autotune(g);
 }
@@ -1185,8 +1316,8 @@ int wasl_hip_engine_trace(const wasl_hip_engine *g, int32_t *donors, double *bas
 int wasl_hip_engine_stencil(const wasl_hip_engine *cg, int32_t *neighbors, double *transforms,
                             double *normals, double *check_basis) {
   return guarded([&] {
-    if (!cg || cg->dim != 2)
-      throw std::runtime_error("engine: limiter stencils exist for triangle meshes");
+    if (!cg || !cg->euler())
+      throw std::runtime_error("engine: limiter stencils exist for 2D Euler");
     auto *g = const_cast<wasl_hip_engine *>(cg);
     g->build_stencil_host();
     const whip::LimiterStencil &s = g->stencil;
@@ -1209,6 +1340,12 @@ int wasl_hip_project(wasl_hip_engine *g, int ic, const double *params, int64_t p
   return guarded([&] {
     if (!g)
       throw std::runtime_error("engine: null argument");
+    const bool fits = g->euler() ? ic < whip::IC_SCALAR_PULSE
+                      : g->dim == 1 ? (ic == whip::IC_SCALAR_PULSE || ic == whip::IC_SCALAR_REGIONS)
+                                    : ic >= whip::IC_SCALAR_CIRCLES;
+    if (!fits)
+      throw std::runtime_error("initial state: initial condition " + std::to_string(ic) +
+                               " does not belong to this engine's equation");
     const std::vector<double> U =
         whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, ic, params, param_count);
     HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -1259,7 +1396,7 @@ int wasl_hip_residual(wasl_hip_engine *g, const double *state, double *rhs) {
 
 int wasl_hip_limit(wasl_hip_engine *g, uint8_t *flags) {
   return guarded([&] {
-    if (g->dim != 2)
+    if (!g->euler())
       throw std::runtime_error("engine: the limiter applies to 2D Euler");
     if (g->p == 0) {
       if (flags)
@@ -1285,7 +1422,7 @@ int wasl_hip_limit(wasl_hip_engine *g, uint8_t *flags) {
 int wasl_hip_stable_dt(wasl_hip_engine *g, double cfl, double *dt) {
   return guarded([&] {
     const double speed = resident_speed(*g);
-    if (g->dim == 1) {
+    if (!g->euler()) {  // cfl_timestep.cpp:53-68
       *dt = speed == 0.0 ? 0.0 : cfl * g->hmin / (speed * (2.0 * g->p + 1.0));
       return;
     }
@@ -1321,16 +1458,16 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     h->cfl_length = cfl * g->hmin;
     h->order_factor = g->dim == 2 ? static_cast<double>(2 * g->p + 1) : (2.0 * g->p + 1.0);
     h->max_steps = max_steps;
-    h->fixed_speed = g->dim == 1 ? 1 : 0;
+    h->fixed_speed = g->euler() ? 0 : 1;
     h->s[0].time = *time;
     h->s[0].step = *step;
-    if (g->dim == 1) {
-      const double a = std::fabs(g->num.wavespeed);
+    if (!g->euler()) {
+      const double a = advection_speed(*g);
       std::memcpy(&h->s[0].lambda, &a, sizeof(double));
     }
     HIPCHECK(hipMemcpyAsync(g->ctl.p, h, sizeof(*h), hipMemcpyHostToDevice, g->stream));
     whip::RunCtl *dctl = g->ctl.as<whip::RunCtl>();
-    if (g->dim == 2)
+    if (g->euler())
       hipLaunchKernelGGL(whip::cfl_speed, dim3((g->E + 255) / 256), dim3(256), 0, g->stream,
                          g->u.as<double>(), g->E, g->Np, g->num.heat_capacity_ratio, dctl, 0);
     if (g->mixed())
@@ -1455,7 +1592,7 @@ int wasl_hip_cell_averages(wasl_hip_engine *g, double *averages) {
     HIPCHECK(hipStreamSynchronize(g->stream));
     const double s = 1.0 / std::sqrt(2.0);
     const size_t E = g->E;
-    if (g->dim == 2) {
+    if (g->euler()) {
       for (size_t e = 0; e < E; ++e)
         for (int v = 0; v < 4; ++v)
           averages[e * 4 + v] = s * U[(static_cast<size_t>(v) * g->Np) * E + e];

@@ -1841,6 +1841,166 @@ __global__ __launch_bounds__(256) void line_stage_batch(const BatchLine *cases, 
   line_body<P>(c.mesh, stage_args(c.u, c.w1, c.w2, c.ctl, b), t.y, t.y == 0, tab);
 }
 
+// ------------------------------------------- 2D scalar advection stage ----
+
+// Reference-element tables of the triangle advection stage (doubles):
+// differentiation matrices Sx[m][n] = sum_q w_q dphi_m/dxi(q) phi_n(q) and
+// Sy (d/deta) over the order-p volume rule, then the edge tables laid out as
+// in Tri<P> (own-side basis, neighbour-side basis, edge weights).
+template <int P> struct TriAdv {
+  static constexpr int Np = (P + 1) * (P + 2) / 2;
+  static constexpr int NE = P + 1;
+  static constexpr int kSx = 0;
+  static constexpr int kSy = Np * Np;
+  static constexpr int kBl = 2 * Np * Np;            // [3][NE][Np]
+  static constexpr int kBr = kBl + 3 * NE * Np;      // [3][3][2][NE][Np]
+  static constexpr int kWe = kBr + 18 * NE * Np;     // [3][NE]
+  static constexpr int kTable = kWe + 3 * NE;
+};
+
+struct AdvectMesh {
+  int E;
+  const double *vel;      // [2][E]: (A vx + B vy) / detJ, (C vx + D vy) / detJ
+  const int *nbr;         // [3][E] neighbour element or -1
+  const int *meta;        // [3][E] edge records (as for Euler)
+  const double *flux;     // [3][E]: (v . n) * edge scale / detJ, n the face normal (left side)
+  const int *bdonor;      // [NB][NE] periodic donors
+  const double *bbasis;   // [NB][NE][Np]
+  const double *table;    // TriAdv<P>::kTable doubles
+};
+
+// One lane per element, state U[element][mode]
+// (dg/scalar_advection/triangle_residual.hpp:25-105 for a constant velocity).
+// The volume term sum_q w_q u_q (v . grad phi_m) is linear in the
+// coefficients: with the contravariant velocity (a_xi, a_eta) = detJ J^-1 v
+// it is a_xi (Sx c)_m + a_eta (Sy c)_m. Faces: the upwind flux
+// (v . n) u_upwind at the left side's edge points, lifted onto this
+// element's own basis; 1/detJ is folded into the velocity and edge factors.
+template <int P>
+__device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageArgs &a,
+                                            const int block, const bool leader, double *tab) {
+  using D = TriAdv<P>;
+  constexpr int Np = D::Np, NE = D::NE;
+  const int E = mesh.E;
+  const int e_raw = block * 256 + static_cast<int>(threadIdx.x);
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+
+  // Global requests ahead of the step-control read.
+  double c[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m)
+    c[m] = a.uin[static_cast<size_t>(e) * Np + m];
+  const double ax = at_row(mesh.vel, E, 0, e), ay = at_row(mesh.vel, E, 1, e);
+  constexpr int kPer = (D::kTable + 255) / 256;
+  double tpre[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    tpre[i] = idx < D::kTable ? mesh.table[idx] : 0.0;
+  }
+
+  double dt = a.dt;
+  if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
+    return;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int idx = static_cast<int>(threadIdx.x) + 256 * i;
+    if (idx < D::kTable)
+      tab[idx] = tpre[i];
+  }
+  __syncthreads();
+
+  double acc[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    double sx = 0.0, sy = 0.0;
+    if constexpr (P > 0) {
+#pragma unroll
+      for (int n = 0; n < Np; ++n) {
+        sx += tab[D::kSx + m * Np + n] * c[n];
+        sy += tab[D::kSy + m * Np + n] * c[n];
+      }
+    }
+    acc[m] = ax * sx + ay * sy;
+  }
+
+  // One edge at a time (the registers of one neighbour), many waves per SIMD
+  // hiding the neighbour loads.
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    const int mk = at_row(mesh.meta, E, k, e);
+    const int nk = at_row(mesh.nbr, E, k, e);
+    const double fk = at_row(mesh.flux, E, k, e);
+    // the neighbour's coefficients (own again across a boundary face: unused)
+    const size_t src = static_cast<size_t>(nk >= 0 ? nk : e);
+    double nc[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      nc[m] = a.uin[src * Np + m];
+    const bool is_left = mk & kMetaLeft;
+    const bool boundary = mk & kMetaBoundary;
+    const int kl = meta_kl(mk), kr = meta_kr(mk), fl = meta_flip(mk);
+    const double *bl = tab + D::kBl + kl * NE * Np;
+    const double *br = tab + D::kBr + ((kl * 3 + kr) * 2 + fl) * NE * Np;
+    const double *own = is_left ? bl : br;
+    const double *oth = is_left ? br : bl;
+    // at P3 one edge point at a time keeps the spill-free register budget
+    constexpr int kUnrollQ = P >= 3 ? 1 : NE;
+#pragma unroll kUnrollQ
+    for (int q = 0; q < NE; ++q) {
+      double si = 0.0, so = 0.0;
+#pragma unroll
+      for (int m = 0; m < Np; ++m) {
+        si += c[m] * own[q * Np + m];
+        so += nc[m] * oth[q * Np + m];
+      }
+      if (boundary) {
+        const int slot = meta_slot(mk);
+        const size_t d = static_cast<size_t>(mesh.bdonor[slot * NE + q]);
+        const double *db = mesh.bbasis + (slot * NE + q) * Np;
+        so = 0.0;
+#pragma unroll
+        for (int m = 0; m < Np; ++m)
+          so += a.uin[d * Np + m] * db[m];
+      }
+      const double L = is_left ? si : so, R = is_left ? so : si;
+      const double f = fk * (fk >= 0.0 ? L : R);
+      const double wq = tab[D::kWe + kl * NE + q];
+      const double sw = is_left ? -wq : wq;
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        acc[m] += sw * own[q * Np + m] * f;
+    }
+  }
+
+#pragma unroll
+  for (int m = 0; m < Np; ++m) {
+    const size_t idx = static_cast<size_t>(e) * Np + m;
+    double y;
+    if (a.residual_only) {
+      y = acc[m];
+    } else {
+      y = a.cb * (c[m] + dt * acc[m]);
+      if (a.ca != 0.0)
+        y += a.ca * a.un[idx];
+    }
+    if (valid)
+      a.out[idx] = y;
+  }
+  stage_epilogue(a.ctl, a.parity, a.stage, dt, leader);
+}
+
+// Up to P2 four waves per SIMD fit spill-free in 128 registers; P3 takes three.
+template <int P> constexpr int advect_waves() { return P >= 3 ? 3 : 4; }
+
+template <int P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect_waves<P>())))
+void advect_stage(AdvectMesh mesh, StageArgs a) {
+  __shared__ double tab[TriAdv<P>::kTable];
+  advect_body<P>(mesh, a, xcd_block(), blockIdx.x == 0, tab);
+}
+
 // ------------------------------------------------------- reductions ----
 
 // CFL speed of the cell averages of U[4][Np][E] into ctl->s[parity].lambda

@@ -772,6 +772,28 @@ double scalar_value(int ic, const double *prm, int64_t np, double x) {
                            std::to_string(ic));
 }
 
+// 2D scalar initial values: the first containing circle, else the default
+// (src/case/initial_regions.cpp:28-34), or a plane sine wave.
+double scalar_value_2d(int ic, const double *prm, int64_t np, double x, double y) {
+  if (ic == IC_SCALAR_CIRCLES) {
+    if (np < 1)
+      throw std::runtime_error("initial state: scalar circles need a default value");
+    for (int64_t k = 1; k + 4 <= np; k += 4) {
+      const double dx = x - prm[k], dy = y - prm[k + 1];
+      if (dx * dx + dy * dy <= prm[k + 2] * prm[k + 2])
+        return prm[k + 3];
+    }
+    return prm[0];
+  }
+  if (ic == IC_SCALAR_WAVE) {
+    if (np < 3)
+      throw std::runtime_error("initial state: a plane wave needs L, kx, ky");
+    return std::sin(2.0 * kPi * (prm[1] * x + prm[2] * y) / prm[0]);
+  }
+  throw std::runtime_error("initial state: unknown 2D scalar initial condition " +
+                           std::to_string(ic));
+}
+
 }  // namespace
 
 // Orthonormal basis: c_m = sum_q w_q f(x_q) phi_m(q), over-integrated with the
@@ -780,6 +802,30 @@ double scalar_value(int ic, const double *prm, int64_t np, double x) {
 std::vector<double> project(const Mesh &m, int p, double gamma, int ic, const double *prm,
                             int64_t np) {
   const size_t E = m.elements();
+  if (m.dimension == 2 && ic >= IC_SCALAR_PULSE) {
+    // project_triangle_field of one variable, transposed to U[E][Np]
+    // (initial_state.cpp:39-46, scalar_coefficient_layout.cpp:14-21).
+    const int Np = tri_modes(p);
+    const TriRule rule = triangle_rule(std::max(p, 5));
+    std::vector<double> basis(static_cast<size_t>(rule.size()) * Np);
+    for (int q = 0; q < rule.size(); ++q)
+      dubiner(rule.xi[q], rule.eta[q], p, basis.data() + q * Np);
+    scalar_value_2d(ic, prm, np, 0.0, 0.0);  // validate once on this thread
+    std::vector<double> U(static_cast<size_t>(Np) * E, 0.0);
+    parallel_for(E, [&](size_t begin, size_t end) {
+      for (size_t e = begin; e < end; ++e) {
+        const Triangle &t = m.tris[e];
+        for (int q = 0; q < rule.size(); ++q) {
+          double x, y;
+          t.to_physical(rule.xi[q], rule.eta[q], x, y);
+          const double f = scalar_value_2d(ic, prm, np, x, y);
+          for (int k = 0; k < Np; ++k)
+            U[e * Np + k] += rule.w[q] * f * basis[q * Np + k];
+        }
+      }
+    });
+    return U;
+  }
   if (m.dimension == 2) {
     const int Np = tri_modes(p);
     const TriRule rule = triangle_rule(std::max(p, 5));

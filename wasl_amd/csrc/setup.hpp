@@ -135,8 +135,9 @@ double minimum_length(const Mesh &mesh);
 // ------------------------------------------------------------- projection ----
 // Initial-condition codes shared with the C ABI.
 enum { IC_EULER_REGIONS = 0, IC_VORTEX = 1, IC_RAREFACTION = 2, IC_SMOOTH = 3,
-       IC_SCALAR_PULSE = 10, IC_SCALAR_REGIONS = 11 };
-// Euler: U[4][Np][E]; scalar: U[E][Np].
+       IC_SCALAR_PULSE = 10, IC_SCALAR_REGIONS = 11, IC_SCALAR_CIRCLES = 12,
+       IC_SCALAR_WAVE = 13 };
+// Euler (codes 0-3): U[4][Np][E]; scalar (10-13): U[E][Np].
 std::vector<double> project(const Mesh &mesh, int p, double gamma, int ic, const double *params,
                             int64_t param_count);
 

@@ -24,7 +24,7 @@ PERIODIC, INFLOW, OUTFLOW = 0, 1, 2
 SCALAR_ADVECTION, EULER = 0, 1
 FP64, MIXED, BF16 = 0, 1, 2
 IC_EULER_REGIONS, IC_VORTEX, IC_RAREFACTION, IC_SMOOTH = 0, 1, 2, 3
-IC_SCALAR_PULSE, IC_SCALAR_REGIONS = 10, 11
+IC_SCALAR_PULSE, IC_SCALAR_REGIONS, IC_SCALAR_CIRCLES, IC_SCALAR_WAVE = 10, 11, 12, 13
 
 OK, ERROR, SIMULATION, DEVICE = 0, -1, -2, -3
 
@@ -56,6 +56,7 @@ class Numerics(C.Structure):
         ("precision", C.c_int32),
         ("lanes_per_element", C.c_int32),
         ("stage_kernel", C.c_int32),
+        ("velocity_y", C.c_double),
     ]
 
 
@@ -245,7 +246,8 @@ class Mesh:
         if count == 0:
             prm = np.zeros(1)
         if self.dimension == 2:
-            n = 4 * (order + 1) * (order + 2) // 2 * self.elements
+            V = 1 if ic >= IC_SCALAR_PULSE else 4
+            n = V * (order + 1) * (order + 2) // 2 * self.elements
         else:
             n = (order + 1) * self.elements
         out = np.zeros(n)
@@ -261,7 +263,7 @@ class Engine:
                  boundary_values=(0.0, 0.0), limiting: bool = False,
                  sensor_threshold: float = 1e-3, density_floor: float = 1e-10,
                  pressure_floor: float = 1e-10, precision: int = FP64, lanes: int = 0,
-                 stage_kernel: int = 0):
+                 stage_kernel: int = 0, velocity_y: float = 0.0):
         lib = library()
         n = Numerics()
         lib.wasl_hip_numerics_default(C.byref(n))
@@ -278,6 +280,7 @@ class Engine:
         n.precision = precision
         n.lanes_per_element = lanes
         n.stage_kernel = stage_kernel
+        n.velocity_y = velocity_y
         self.numerics = n
         h = _P()
         _check(lib.wasl_hip_engine_create(mesh.h, C.byref(n), C.byref(h)))
@@ -391,7 +394,7 @@ class Engine:
         return m.value
 
     def cell_averages(self) -> np.ndarray:
-        out = np.zeros(self.E * (4 if self.dimension == 2 else 1))
+        out = np.zeros(self.E * self.V)
         _check(library().wasl_hip_cell_averages(self.h, out))
         return out.reshape(self.E, -1)
 
