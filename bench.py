@@ -34,12 +34,15 @@ FP64_PEAK_TFLOPS = 78.6
 # FETCH_SIZE and WRITE_SIZE (one pass each) and scaled by what the same
 # counters read for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0); keyed by
 # (kernel, order, triangles, precision). The commands and counter values are in
-# profiles/euler_stage_p2_32k_fp64.md and profiles/euler_stage_mx_p3_524k.md.
+# profiles/euler_stage_p2_32k_fp64.md, profiles/euler_stage_mx_p3_524k.md and
+# profiles/advect_stage_524k.md.
 # Other workloads report null.
 MEASURED_TRAFFIC = {
     ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7895.22 * 2.0 + 6149.39) * 1024.0,
     ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7965.57 * 2.0 + 6146.72) * 1024.0,
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (230474.0 * 2.0 + 262229.0) * 1024.0,
+    ("whip::advect_stage<1>", 1, 524288, "fp64"): (26979.5 * 2.0 + 12288.1) * 1024.0,
+    ("whip::advect_stage<3>", 3, 524288, "fp64"): (51864.7 * 2.0 + 40960.1) * 1024.0,
 }
 
 
@@ -184,6 +187,97 @@ def rarefaction_run(cells: int, end_time: float) -> dict:
     return out
 
 
+def advect_bytes_per_element_stage(Np: int) -> float:
+    """The 2D scalar advection stage reads its input and writes its output
+    (8 Np B each), reads U^n in two stages of three (8 Np B x 2/3), and reads
+    the contravariant velocity (16 B), the edge flux factors (24 B) and the
+    edge records (neighbour and meta, 24 B). The neighbours' coefficients are
+    the same array as the input and count once."""
+    return 8 * Np * (2 + 2 / 3) + 64
+
+
+def advect2d_run(args) -> int:
+    """2D scalar advection u_t + div(v u) = 0, v = (0.7, -0.4), plane wave
+    sin(2 pi (x + 2 y) / 10) on periodic [0,10]^2, 2 cells^2 jittered
+    triangles, P = --order, CFL 0.3, FP64 (scalar_advection_triangle.cpp).
+    value = DOF-updates/s of the stepping loop."""
+    import torch
+
+    import wasl_amd
+    from wasl_amd import meshgen as MG
+
+    p, cfl, vel, wave = args.order, 0.3, (0.7, -0.4), (10.0, 1.0, 2.0)
+    mesh_in = MG.jittered_triangles(args.cells, (0.0, 0.0), (10.0, 10.0), 0.15)
+    g = wasl_amd.Engine(wasl_amd.Mesh.from_input(mesh_in), wasl_amd.SCALAR_ADVECTION, p,
+                        wavespeed=vel[0], velocity_y=vel[1])
+    g.project(wasl_amd.IC_SCALAR_WAVE, wave)
+    E, Np = g.E, g.Np
+    t, s, _ = g.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                        stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = E * Np * args.steps / wall
+    g.set_timing(True)
+    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                        stop_at_max_steps=True)
+    g.set_timing(False)
+    _, stage_total, launches = g.last_timing()
+    stage_ms = stage_total / max(launches, 1)
+    bytes_stage = E * advect_bytes_per_element_stage(Np)
+    achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
+    log(f"advect2d P{p}: {E} triangles, {wall * 1e3 / args.steps:.4f} ms/step, "
+        f"{value:.4e} DOF-updates/s, stage {stage_ms * 1e3:.2f} us, {achieved:.0f} GB/s")
+    traffic = MEASURED_TRAFFIC.get((f"whip::advect_stage<{p}>", p, E, "fp64"))
+    result = {
+        "metric": "2D scalar advection DOF-updates/s",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D scalar advection, v = {vel}, plane wave, P{p} modal Dubiner, "
+                        f"2x{args.cells}x{args.cells}={E} jittered periodic triangles on "
+                        f"[0,10]^2, upwind, SSP-RK3, CFL {cfl}, FP64",
+            "elements": E, "modes": Np, "dofs": E * Np,
+            "state_finite": bool(np.isfinite(g.state).all()),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_frac_of_algorithmic": None if traffic is None else traffic / bytes_stage,
+            "kernel": f"whip::advect_stage<{p}>", "stage_kernel_us": stage_ms * 1e3,
+            "algorithmic_bytes_per_launch": bytes_stage,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        kind = "reference" if O.available("reference") else "port"
+        c = O.tri_scalar(mesh_in.node_tags, mesh_in.node_xy, mesh_in.triangles, p, *vel,
+                         kind=kind)
+        U = c.project(O.IC_SCALAR_WAVE, wave)
+        dt = c.stable_dt(U, cfl)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            U = c.step(U, dt)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > args.cpu_seconds or steps >= 200:
+                break
+        result["cpu_baseline"] = {
+            "value": c.E * c.Np * steps / el, "unit": "DOF-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{steps} SSP-RK3 steps of the same P{p} {c.E}-triangle advection case "
+                      f"({el:.1f} s, single thread, {cpu_model()})",
+        }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
@@ -276,7 +370,9 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
+    # 64 warmup steps are one full captured batch: the step graph is recorded
+    # and instantiated before the timed region.
+    ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--cells", type=int, default=128, help="quads per axis (2 n^2 triangles)")
     ap.add_argument("--lanes", type=int, default=0)
@@ -288,9 +384,11 @@ def main(argv=None) -> int:
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests of the serving mix (0: skip the serving measurement)")
     ap.add_argument("--serving-seed", type=int, default=0x5EED)
-    ap.add_argument("--workload", choices=["vortex", "rarefaction"], default="vortex",
+    ap.add_argument("--workload", choices=["vortex", "rarefaction", "advect2d"], default="vortex",
                     help="vortex: the headline (config 2 sizes by default); rarefaction: the "
-                         "limited radial rarefaction run to --end-time, one JSON line of its own")
+                         "limited radial rarefaction run to --end-time; advect2d: 2D scalar "
+                         "advection at --order and --cells; each other workload prints one "
+                         "JSON line of its own")
     ap.add_argument("--end-time", type=float, default=0.3)
     ap.add_argument("--error-steps", type=int, default=20,
                     help="steps of the reduced-precision error check (--precision mixed/bf16)")
@@ -325,6 +423,9 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
         del src, dst
+
+    if args.workload == "advect2d":
+        return advect2d_run(args)
 
     if args.workload == "rarefaction":
         res = rarefaction_run(args.cells, args.end_time)

@@ -172,6 +172,31 @@ struct wasl_hip_engine {
 
   bool mixed() const { return euler() && num.precision != WASL_HIP_FP64; }
 
+  // The 2D scalar state lives on the device as U[mode][element]; the C ABI
+  // speaks the reference's U[element][mode] (triangle_residual.hpp:3-4).
+  bool mode_major_scalar() const { return !euler() && dim == 2; }
+
+  // Host layout -> device layout (a copy in `tmp` when they differ).
+  const double *to_device(const double *host, std::vector<double> &tmp) const {
+    if (!mode_major_scalar())
+      return host;
+    tmp.resize(state_size());
+    for (int e = 0; e < E; ++e)
+      for (int m = 0; m < Np; ++m)
+        tmp[static_cast<size_t>(m) * E + e] = host[static_cast<size_t>(e) * Np + m];
+    return tmp.data();
+  }
+  // Device layout (in `dev`) -> host layout in `host`.
+  void to_host(const double *dev, double *host) const {
+    if (!mode_major_scalar()) {
+      std::copy(dev, dev + state_size(), host);
+      return;
+    }
+    for (int e = 0; e < E; ++e)
+      for (int m = 0; m < Np; ++m)
+        host[static_cast<size_t>(e) * Np + m] = dev[static_cast<size_t>(m) * E + e];
+  }
+
   // The trace buffer that belongs to a state buffer (mixed precision only).
   float *traces_of(const double *p) const {
     if (!mixed())
@@ -638,7 +663,7 @@ void raise_device_error(wasl_hip_engine &g) {
 double mass_of(wasl_hip_engine &g) {
   const int blocks = (g.E + 255) / 256;
   hipLaunchKernelGGL(whip::mass_partials, dim3(blocks), dim3(256), 0, g.stream,
-                     g.u.as<double>(), g.euler() ? 1 : g.Np, g.mass_w.as<double>(), g.E,
+                     g.u.as<double>(), g.dim == 2 ? 1 : g.Np, g.mass_w.as<double>(), g.E,
                      g.partials.as<double>());
   hipLaunchKernelGGL(whip::mass_finish, dim3(1), dim3(64), 0, g.stream, g.partials.as<double>(),
                      blocks, g.result.as<double>());
@@ -968,7 +993,13 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     upload(g.meta, rec.meta, tot, g.stream);
     upload(g.bdonor, rec.bdon, tot, g.stream);
     upload(g.bbasis, rec.bbas, tot, g.stream);
-    upload(g.table, advect_tables(g.p), tot, g.stream);
+    const std::vector<double> tables = advect_tables(g.p);
+    upload(g.table, tables, tot, g.stream);
+    HIPCHECK(hipMemcpyToSymbolAsync(HIP_SYMBOL(whip::g_advvol), tables.data(),
+                                    2 * g.Np * g.Np * sizeof(double),
+                                    static_cast<size_t>(g.p) * whip::kAdvVolMax * sizeof(double),
+                                    hipMemcpyHostToDevice, g.stream));
+    HIPCHECK(hipStreamSynchronize(g.stream));
     g.hmin = whip::minimum_length(mesh);
     g.stencil.highest_begin = g.Np;
   } else if (num.equation == WASL_HIP_SCALAR_ADVECTION) {
@@ -1348,8 +1379,9 @@ int wasl_hip_project(wasl_hip_engine *g, int ic, const double *params, int64_t p
                                " does not belong to this engine's equation");
     const std::vector<double> U =
         whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, ic, params, param_count);
-    HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
-                            g->stream));
+    std::vector<double> tmp;
+    HIPCHECK(hipMemcpyAsync(g->u.p, g->to_device(U.data(), tmp), U.size() * sizeof(double),
+                            hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
     if (state_out)
       std::copy(U.begin(), U.end(), state_out);
@@ -1358,7 +1390,8 @@ int wasl_hip_project(wasl_hip_engine *g, int ic, const double *params, int64_t p
 
 int wasl_hip_upload_state(wasl_hip_engine *g, const double *state) {
   return guarded([&] {
-    HIPCHECK(hipMemcpyAsync(g->u.p, state, g->state_size() * sizeof(double),
+    std::vector<double> tmp;
+    HIPCHECK(hipMemcpyAsync(g->u.p, g->to_device(state, tmp), g->state_size() * sizeof(double),
                             hipMemcpyHostToDevice, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
   });
@@ -1366,16 +1399,26 @@ int wasl_hip_upload_state(wasl_hip_engine *g, const double *state) {
 
 int wasl_hip_download_state(wasl_hip_engine *g, double *state) {
   return guarded([&] {
-    HIPCHECK(hipMemcpyAsync(state, g->u.p, g->state_size() * sizeof(double),
+    if (!g->mode_major_scalar()) {
+      HIPCHECK(hipMemcpyAsync(state, g->u.p, g->state_size() * sizeof(double),
+                              hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      return;
+    }
+    std::vector<double> dev(g->state_size());
+    HIPCHECK(hipMemcpyAsync(dev.data(), g->u.p, dev.size() * sizeof(double),
                             hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
+    g->to_host(dev.data(), state);
   });
 }
 
 int wasl_hip_residual(wasl_hip_engine *g, const double *state, double *rhs) {
   return guarded([&] {
     const size_t n = g->state_size() * sizeof(double);
-    HIPCHECK(hipMemcpyAsync(g->w1.p, state, n, hipMemcpyHostToDevice, g->stream));
+    std::vector<double> tmp;
+    HIPCHECK(hipMemcpyAsync(g->w1.p, g->to_device(state, tmp), n, hipMemcpyHostToDevice,
+                            g->stream));
     whip::StageArgs a{};
     a.uin = g->w1.as<double>();
     a.un = g->w1.as<double>();
@@ -1389,6 +1432,13 @@ int wasl_hip_residual(wasl_hip_engine *g, const double *state, double *rhs) {
       a.tin = g->tr_w1.as<float>();
     }
     launch_stage(*g, a);
+    if (g->mode_major_scalar()) {
+      std::vector<double> dev(g->state_size());
+      HIPCHECK(hipMemcpyAsync(dev.data(), g->w2.p, n, hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      g->to_host(dev.data(), rhs);
+      return;
+    }
     HIPCHECK(hipMemcpyAsync(rhs, g->w2.p, n, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
   });
@@ -1596,6 +1646,9 @@ int wasl_hip_cell_averages(wasl_hip_engine *g, double *averages) {
       for (size_t e = 0; e < E; ++e)
         for (int v = 0; v < 4; ++v)
           averages[e * 4 + v] = s * U[(static_cast<size_t>(v) * g->Np) * E + e];
+    } else if (g->mode_major_scalar()) {
+      for (size_t e = 0; e < E; ++e)
+        averages[e] = U[e] / std::sqrt(2.0);
     } else {
       for (size_t e = 0; e < E; ++e)
         averages[e] = U[e * g->Np] / std::sqrt(2.0);

@@ -1856,7 +1856,16 @@ template <int P> struct TriAdv {
   static constexpr int kBr = kBl + 3 * NE * Np;      // [3][3][2][NE][Np]
   static constexpr int kWe = kBr + 18 * NE * Np;     // [3][NE]
   static constexpr int kTable = kWe + 3 * NE;
+  // The edge tables alone (staged in LDS; the matrices are read from the
+  // constant slot g_advvol[P] through the scalar cache).
+  static constexpr int kEdge = kBl;
+  static constexpr int kEdgeTable = kTable - kBl;
 };
+
+// Differentiation matrices Sx, Sy of order p in slot p (every engine of that
+// order writes the same values at creation).
+constexpr int kAdvVolMax = 2 * 10 * 10;
+__constant__ double g_advvol[4][kAdvVolMax];
 
 struct AdvectMesh {
   int E;
@@ -1869,8 +1878,11 @@ struct AdvectMesh {
   const double *table;    // TriAdv<P>::kTable doubles
 };
 
-// One lane per element, state U[element][mode]
-// (dg/scalar_advection/triangle_residual.hpp:25-105 for a constant velocity).
+// One lane per element (dg/scalar_advection/triangle_residual.hpp:25-105 for
+// a constant velocity). The device keeps the state as U[mode][element] so
+// every coefficient load and store is coalesced over the elements of a wave;
+// the engine converts from and to the reference's U[element][mode] at the
+// host boundary.
 // The volume term sum_q w_q u_q (v . grad phi_m) is linear in the
 // coefficients: with the contravariant velocity (a_xi, a_eta) = detJ J^-1 v
 // it is a_xi (Sx c)_m + a_eta (Sy c)_m. Faces: the upwind flux
@@ -1886,31 +1898,55 @@ __device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageA
   const bool valid = e_raw < E;
   const int e = valid ? e_raw : E - 1;
 
-  // Global requests ahead of the step-control read.
+  // Global requests ahead of the step-control read: own coefficients, the
+  // contravariant velocity and the three edge records.
   double c[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m)
-    c[m] = a.uin[static_cast<size_t>(e) * Np + m];
+    c[m] = at_row(a.uin, E, m, e);
   const double ax = at_row(mesh.vel, E, 0, e), ay = at_row(mesh.vel, E, 1, e);
-  constexpr int kPer = (D::kTable + 255) / 256;
+  int meta[3], nb[3];
+  double fe[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    meta[k] = at_row(mesh.meta, E, k, e);
+    nb[k] = at_row(mesh.nbr, E, k, e);
+    fe[k] = at_row(mesh.flux, E, k, e);
+  }
+  constexpr int kPer = (D::kEdgeTable + 255) / 256;
   double tpre[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = static_cast<int>(threadIdx.x) + 256 * i;
-    tpre[i] = idx < D::kTable ? mesh.table[idx] : 0.0;
+    tpre[i] = idx < D::kEdgeTable ? mesh.table[D::kEdge + idx] : 0.0;
   }
 
   double dt = a.dt;
   if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
     return;
+
+  // The neighbour's coefficients (own again across a boundary face, where
+  // they go unused), requested one edge ahead of their use.
+  const auto fetch = [&](int k, double (&dst)[Np]) {
+    const int src = nb[k] >= 0 ? nb[k] : e;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      dst[m] = at_row(a.uin, E, m, src);
+  };
+  double na[Np], nbk[Np];
+  fetch(0, na);
+
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = static_cast<int>(threadIdx.x) + 256 * i;
-    if (idx < D::kTable)
+    if (idx < D::kEdgeTable)
       tab[idx] = tpre[i];
   }
   __syncthreads();
 
+  // Volume: matrix entries are wave-uniform, read into scalar registers.
+  static_assert(2 * Np * Np <= kAdvVolMax, "advection matrix slot too small");
+  const double *S = g_advvol[P];
   double acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
@@ -1918,36 +1954,24 @@ __device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageA
     if constexpr (P > 0) {
 #pragma unroll
       for (int n = 0; n < Np; ++n) {
-        sx += tab[D::kSx + m * Np + n] * c[n];
-        sy += tab[D::kSy + m * Np + n] * c[n];
+        sx += S[D::kSx + m * Np + n] * c[n];
+        sy += S[D::kSy + m * Np + n] * c[n];
       }
     }
     acc[m] = ax * sx + ay * sy;
   }
 
-  // One edge at a time (the registers of one neighbour), many waves per SIMD
-  // hiding the neighbour loads.
-#pragma unroll 1
-  for (int k = 0; k < 3; ++k) {
-    const int mk = at_row(mesh.meta, E, k, e);
-    const int nk = at_row(mesh.nbr, E, k, e);
-    const double fk = at_row(mesh.flux, E, k, e);
-    // the neighbour's coefficients (own again across a boundary face: unused)
-    const size_t src = static_cast<size_t>(nk >= 0 ? nk : e);
-    double nc[Np];
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
-      nc[m] = a.uin[src * Np + m];
+  // Upwind flux at the left side's edge points, lifted onto the own basis.
+  const auto edge = [&](int mk, double fk, const double (&nc)[Np]) {
     const bool is_left = mk & kMetaLeft;
     const bool boundary = mk & kMetaBoundary;
     const int kl = meta_kl(mk), kr = meta_kr(mk), fl = meta_flip(mk);
-    const double *bl = tab + D::kBl + kl * NE * Np;
-    const double *br = tab + D::kBr + ((kl * 3 + kr) * 2 + fl) * NE * Np;
+    const double *bl = tab + (D::kBl - D::kEdge) + kl * NE * Np;
+    const double *br = tab + (D::kBr - D::kEdge) + ((kl * 3 + kr) * 2 + fl) * NE * Np;
     const double *own = is_left ? bl : br;
     const double *oth = is_left ? br : bl;
-    // at P3 one edge point at a time keeps the spill-free register budget
-    constexpr int kUnrollQ = P >= 3 ? 1 : NE;
-#pragma unroll kUnrollQ
+    // one edge point at a time keeps the register budget spill-free
+#pragma unroll 1
     for (int q = 0; q < NE; ++q) {
       double si = 0.0, so = 0.0;
 #pragma unroll
@@ -1957,26 +1981,31 @@ __device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageA
       }
       if (boundary) {
         const int slot = meta_slot(mk);
-        const size_t d = static_cast<size_t>(mesh.bdonor[slot * NE + q]);
+        const int d = mesh.bdonor[slot * NE + q];
         const double *db = mesh.bbasis + (slot * NE + q) * Np;
         so = 0.0;
 #pragma unroll
         for (int m = 0; m < Np; ++m)
-          so += a.uin[d * Np + m] * db[m];
+          so += at_row(a.uin, E, m, d) * db[m];
       }
       const double L = is_left ? si : so, R = is_left ? so : si;
       const double f = fk * (fk >= 0.0 ? L : R);
-      const double wq = tab[D::kWe + kl * NE + q];
+      const double wq = tab[(D::kWe - D::kEdge) + kl * NE + q];
       const double sw = is_left ? -wq : wq;
 #pragma unroll
       for (int m = 0; m < Np; ++m)
         acc[m] += sw * own[q * Np + m] * f;
     }
-  }
+  };
+  fetch(1, nbk);
+  edge(meta[0], fe[0], na);
+  fetch(2, na);
+  edge(meta[1], fe[1], nbk);
+  edge(meta[2], fe[2], na);
 
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
-    const size_t idx = static_cast<size_t>(e) * Np + m;
+    const size_t idx = static_cast<size_t>(m) * static_cast<unsigned>(E) + e;
     double y;
     if (a.residual_only) {
       y = acc[m];
@@ -1997,7 +2026,7 @@ template <int P> constexpr int advect_waves() { return P >= 3 ? 3 : 4; }
 template <int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect_waves<P>())))
 void advect_stage(AdvectMesh mesh, StageArgs a) {
-  __shared__ double tab[TriAdv<P>::kTable];
+  __shared__ double tab[TriAdv<P>::kEdgeTable];
   advect_body<P>(mesh, a, xcd_block(), blockIdx.x == 0, tab);
 }
 
