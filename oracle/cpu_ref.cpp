@@ -734,6 +734,7 @@ struct orc_case {
   Vec stage, rhs, euler;
 
   size_t E() const { return dim == 2 ? tris.size() : lines.size(); }
+  bool is_euler() const { return V == 4; }
   size_t F() const { return dim == 2 ? faces.size() : lfaces.size(); }
   size_t state_size() const { return static_cast<size_t>(V) * n * E(); }
 };
@@ -1068,8 +1069,92 @@ void tri_scalar_residual(const orc_case &c, const double *U, const double *ext, 
   }
 }
 
+// src/boundaries/line_euler_exterior.cpp:13-43: exterior per [variable][face]
+Vec line_euler_exterior(const orc_case &c, const double *U) {
+  const size_t E = c.E(), F = c.lfaces.size();
+  Vec bl(c.n), br(c.n);
+  legendre(-1.0, c.p, bl.data());
+  legendre(1.0, c.p, br.data());
+  Vec ext(4 * F, 0.0);
+  for (size_t f = 0; f < F; ++f) {
+    const LineFace &face = c.lfaces[f];
+    if (face.right != -1)
+      continue;
+    const bool left_end = face.normal < 0.0;
+    const int kind = left_end ? c.left_kind : c.right_kind;
+    if (kind == ORC_INFLOW)
+      throw std::invalid_argument("line Euler exterior: inflow is not implemented");
+    const bool periodic = kind == ORC_PERIODIC;
+    const size_t donor = periodic ? (left_end ? E - 1 : 0) : static_cast<size_t>(face.left);
+    const bool right_endpoint = periodic ? left_end : !left_end;
+    const Vec &b = right_endpoint ? br : bl;
+    for (int v = 0; v < 4; ++v)
+      for (int m = 0; m < c.n; ++m)
+        ext[v * F + f] += U[(static_cast<size_t>(v) * c.n + m) * E + donor] * b[m];
+  }
+  return ext;
+}
+
+// include/wasl/dg/euler/line_residual.hpp:23-86
+void line_euler_residual(const orc_case &c, const double *U, const double *ext, double *rhs) {
+  const int n = c.n;
+  const size_t E = c.E(), F = c.lfaces.size();
+  const Rule vol = gauss(std::max(1, c.p + 1));
+  std::fill(rhs, rhs + 4 * n * E, 0.0);
+  Vec b(n), g(n);
+  for (size_t e = 0; e < E; ++e) {
+    for (int q = 0; q < vol.size(); ++q) {
+      legendre(vol.x[q], c.p, b.data());
+      legendre_gradient(vol.x[q], c.p, g.data());
+      const Cons fx = flux_x(c.gamma, eval_state(U, n, E, e, b.data()));
+      for (int v = 0; v < 4; ++v)
+        for (int m = 0; m < n; ++m)
+          rhs[(static_cast<size_t>(v) * n + m) * E + e] += vol.w[q] * fx[v] * g[m];
+    }
+  }
+  Vec bl(n), br(n);
+  legendre(-1.0, c.p, bl.data());
+  legendre(1.0, c.p, br.data());
+  for (size_t f = 0; f < F; ++f) {
+    const LineFace &face = c.lfaces[f];
+    const size_t L = static_cast<size_t>(face.left);
+    const bool owned_at_left_end = face.normal < 0.0;
+    const Vec &ob = owned_at_left_end ? bl : br;
+    const Cons owner = eval_state(U, n, E, L, ob.data());
+    Cons exterior{0.0, 0.0, 0.0, 0.0};
+    if (face.right != -1) {
+      exterior = eval_state(U, n, E, static_cast<size_t>(face.right), bl.data());
+    } else {
+      for (int v = 0; v < 4; ++v)
+        exterior[v] = ext[v * F + f];
+    }
+    const Cons flux = owned_at_left_end ? hllc(c.gamma, exterior, owner, 1.0, 0.0)
+                                        : hllc(c.gamma, owner, exterior, 1.0, 0.0);
+    for (int v = 0; v < 4; ++v) {
+      const double value = flux[v];
+      for (int m = 0; m < n; ++m)
+        rhs[(static_cast<size_t>(v) * n + m) * E + L] +=
+            (owned_at_left_end ? 1.0 : -1.0) * ob[m] * value;
+      if (face.right != -1) {
+        const size_t R = static_cast<size_t>(face.right);
+        for (int m = 0; m < n; ++m)
+          rhs[(static_cast<size_t>(v) * n + m) * E + R] += bl[m] * value;
+      }
+    }
+  }
+  for (size_t e = 0; e < E; ++e) {
+    const double factor = 1.0 / ((c.lines[e].xr - c.lines[e].xl) / 2.0);
+    for (int v = 0; v < 4; ++v)
+      for (int m = 0; m < n; ++m)
+        rhs[(static_cast<size_t>(v) * n + m) * E + e] *= factor;
+  }
+}
+
 void residual(const orc_case &c, const double *U, double *rhs) {
-  if (c.scalar) {
+  if (c.dim == 1 && c.is_euler()) {
+    const Vec ext = line_euler_exterior(c, U);
+    line_euler_residual(c, U, ext.data(), rhs);
+  } else if (c.scalar) {
     const Vec ext = tri_scalar_exterior(c, U);
     tri_scalar_residual(c, U, ext.data(), rhs);
   } else if (c.dim == 2) {
@@ -1079,6 +1164,63 @@ void residual(const orc_case &c, const double *U, double *rhs) {
     const Vec ext = line_exterior(c, U);
     line_scalar_residual(c, U, ext.data(), rhs);
   }
+}
+
+// src/limiting/euler_limiter_stencil_line.cpp:14-94 (the neighbour slots,
+// normals (1, 0) and the high-mode start are set by the caller).
+void build_line_stencil(orc_case &c) {
+  Stencil &s = c.stencil;
+  const size_t E = c.E();
+  int32_t left_boundary = -1, right_boundary = -1;
+  for (const LineFace &face : c.lfaces) {
+    if (face.right != -1) {
+      append_unique(s.nbrs, s.K, static_cast<size_t>(face.left), face.right);
+      append_unique(s.nbrs, s.K, static_cast<size_t>(face.right), face.left);
+    } else if (face.normal < 0.0) {
+      left_boundary = face.left;
+    } else {
+      right_boundary = face.left;
+    }
+  }
+  if (c.left_kind == ORC_PERIODIC && left_boundary != -1 && right_boundary != -1) {
+    append_unique(s.nbrs, s.K, static_cast<size_t>(left_boundary), right_boundary);
+    append_unique(s.nbrs, s.K, static_cast<size_t>(right_boundary), left_boundary);
+  }
+  const Rule rule = gauss(std::max(1, c.p + 1));
+  const double period = c.lines.back().xr - c.lines.front().xl;
+  Vec sb(c.n), tb(c.n);
+  for (size_t e = 0; e < E; ++e) {
+    const double txl = c.lines[e].xl, txr = c.lines[e].xr;
+    const double tcentre = (txl + txr) / 2.0, tjac = (txr - txl) / 2.0;
+    for (int k = 0; k < s.K; ++k) {
+      const int32_t nb = s.nbrs[e * s.K + k];
+      if (nb == -1)
+        continue;
+      double sxl = c.lines[static_cast<size_t>(nb)].xl, sxr = c.lines[static_cast<size_t>(nb)].xr;
+      const double shift = periodic_offset(tcentre - (sxl + sxr) / 2.0, period);
+      sxl += shift;
+      sxr += shift;
+      const double scentre = (sxl + sxr) / 2.0, sjac = (sxr - sxl) / 2.0;
+      double *M = s.transforms.data() + (e * s.K + k) * c.n * c.n;
+      for (int q = 0; q < rule.size(); ++q) {
+        const double sxi = (tcentre + tjac * rule.x[q] - scentre) / sjac;
+        legendre(sxi, c.p, sb.data());
+        legendre(rule.x[q], c.p, tb.data());
+        for (int tm = 0; tm < c.n; ++tm)
+          for (int sm = 0; sm < c.n; ++sm)
+            M[tm * c.n + sm] += rule.w[q] * tb[tm] * sb[sm];
+      }
+    }
+  }
+  Vec b(c.n);
+  const auto append = [&](double xi) {
+    legendre(xi, c.p, b.data());
+    s.check.insert(s.check.end(), b.begin(), b.end());
+  };
+  for (int q = 0; q < rule.size(); ++q)
+    append(rule.x[q]);
+  append(-1.0);
+  append(1.0);
 }
 
 // src/limiting/euler_limiter_stencil_triangle.cpp:18-112
@@ -1094,6 +1236,8 @@ void build_stencil(orc_case &c) {
   s.transforms.assign(E * s.K * c.n * c.n, 0.0);
   s.check.clear();
   if (c.dim != 2) {
+    if (c.is_euler())
+      build_line_stencil(c);
     c.has_stencil = true;
     return;
   }
@@ -1227,6 +1371,24 @@ void limit(const orc_case &c, double thr, double rf, double pf, double *U, uint8
 
 // src/time/cfl_timestep.cpp:14-84
 double stable_dt(const orc_case &c, const double *U, double cfl) {
+  if (c.dim == 1 && c.is_euler()) {  // :14-19, :33-53, :70-76 with |u_x|
+    const size_t E = c.E();
+    double h = std::numeric_limits<double>::max();
+    for (const auto &l : c.lines)
+      h = std::min(h, l.xr - l.xl);
+    const double scale = 1.0 / std::sqrt(2.0);
+    double speed = 0.0;
+    for (size_t e = 0; e < E; ++e) {
+      const Cons avg{scale * U[e], scale * U[static_cast<size_t>(c.n) * E + e],
+                     scale * U[2 * static_cast<size_t>(c.n) * E + e],
+                     scale * U[3 * static_cast<size_t>(c.n) * E + e]};
+      if (!is_physical(c.gamma, avg))
+        throw std::runtime_error("simulation: Euler cell average is not physical");
+      const Prim w = primitive(c.gamma, avg);
+      speed = std::max(speed, std::fabs(w.u) + sound_speed(c.gamma, w));
+    }
+    return speed > 0.0 ? cfl * h / (speed * (2 * c.p + 1)) : 0.0;
+  }
   if (c.dim == 1) {
     if (c.lines.empty() || c.wavespeed == 0.0)
       return 0.0;
@@ -1269,7 +1431,7 @@ void step(orc_case &c, double dt, int limiting, double thr, double rf, double pf
   c.rhs.resize(N);
   c.euler.resize(N);
   const auto post = [&](double *s) {
-    if (limiting && c.dim == 2 && !c.scalar)
+    if (limiting && c.is_euler())
       limit(c, thr, rf, pf, s, nullptr);
   };
   residual(c, U, c.rhs.data());
@@ -1295,6 +1457,9 @@ double mass(const orc_case &c, const double *U) {
   } else if (c.dim == 2) {
     for (size_t e = 0; e < c.E(); ++e)
       r += std::sqrt(2.0) * c.tris[e].map().jacobian_determinant() * U[e];
+  } else if (c.is_euler()) {  // euler_diagnostics.cpp:22-27
+    for (size_t e = 0; e < c.E(); ++e)
+      r += std::sqrt(2.0) * ((c.lines[e].xr - c.lines[e].xl) / 2.0) * U[e];
   } else {
     for (size_t e = 0; e < c.E(); ++e)
       r += std::sqrt(2.0) * ((c.lines[e].xr - c.lines[e].xl) / 2.0) * U[e * c.n];
@@ -1428,6 +1593,24 @@ void project(const orc_case &c, int ic, const double *prm, int64_t np, double *U
     }
     return;
   }
+  if (c.is_euler()) {  // initial_state.cpp:48-54: U[variable][mode][element]
+    const Rule rule = gauss(7);
+    std::fill(U, U + 4 * c.n * E, 0.0);
+    Vec b(c.n);
+    for (size_t e = 0; e < E; ++e) {
+      const double xl = c.lines[e].xl, xr = c.lines[e].xr;
+      for (int q = 0; q < rule.size(); ++q) {
+        legendre(rule.x[q], c.p, b.data());
+        const double centre = (xl + xr) / 2.0, jac = (xr - xl) / 2.0;
+        const Cons s = conserved(c.gamma, euler_ic(ic, prm, np, c.gamma, centre + jac * rule.x[q],
+                                                   0.0));
+        for (int v = 0; v < 4; ++v)
+          for (int m = 0; m < c.n; ++m)
+            U[(static_cast<size_t>(v) * c.n + m) * E + e] += rule.w[q] * s[v] * b[m];
+      }
+    }
+    return;
+  }
   // project_line_field writes U[variable][mode][element]; with one variable
   // the transpose to the solver layout is U[element][mode].
   const Rule rule = gauss(7);
@@ -1445,6 +1628,39 @@ void project(const orc_case &c, int ic, const double *prm, int64_t np, double *U
   for (size_t e = 0; e < E; ++e)
     for (int m = 0; m < c.n; ++m)
       U[e * c.n + m] = tmp[static_cast<size_t>(m) * E + e];
+}
+
+// src/mesh/line/build_line_mesh.cpp:15-85: elements oriented left to right
+// and sorted by their left end, faces from left to right.
+void build_lines(orc_case &c, int64_t node_count, const int64_t *node_tags, const double *node_x,
+                 int64_t segment_count, const int64_t *segment_nodes) {
+  std::map<int64_t, int64_t> lookup;
+  for (int64_t i = 0; i < node_count; ++i)
+    lookup[node_tags[i]] = i;
+  for (int64_t s = 0; s < segment_count; ++s) {
+    const int64_t ta = segment_nodes[2 * s], tb = segment_nodes[2 * s + 1];
+    const auto ia = lookup.find(ta), ib = lookup.find(tb);
+    if (ia == lookup.end() || ib == lookup.end())
+      throw std::runtime_error("line mesh: missing node");
+    const double xa = node_x[ia->second], xb = node_x[ib->second];
+    if (xa == xb)
+      throw std::runtime_error("line mesh: zero-length element " + std::to_string(s + 1));
+    const bool fwd = xa < xb;
+    c.lines.push_back(fwd ? LineElem{ta, tb, xa, xb} : LineElem{tb, ta, xb, xa});
+  }
+  if (c.lines.empty())
+    throw std::runtime_error("line mesh: document contains no line elements");
+  std::stable_sort(c.lines.begin(), c.lines.end(),
+                   [](const LineElem &a, const LineElem &b) { return a.xl < b.xl; });
+  c.lfaces.push_back({0, -1, c.lines.front().xl, -1.0});
+  for (size_t i = 0; i + 1 < c.lines.size(); ++i) {
+    if (c.lines[i].rn != c.lines[i + 1].ln)
+      throw std::runtime_error("line mesh: gap or discontinuity between elements " +
+                               std::to_string(i) + " and " + std::to_string(i + 1));
+    c.lfaces.push_back(
+        {static_cast<int32_t>(i), static_cast<int32_t>(i + 1), c.lines[i].xr, 1.0});
+  }
+  c.lfaces.push_back({static_cast<int32_t>(c.lines.size() - 1), -1, c.lines.back().xr, 1.0});
 }
 
 template <typename F> int guarded(F &&f) {
@@ -1531,34 +1747,35 @@ orc_case *orc_line_scalar_create(int64_t node_count, const int64_t *node_tags,
         c->right_kind = right_kind;
         c->left_value = left_value;
         c->right_value = right_value;
-        std::map<int64_t, int64_t> lookup;
-        for (int64_t i = 0; i < node_count; ++i)
-          lookup[node_tags[i]] = i;
-        for (int64_t s = 0; s < segment_count; ++s) {
-          const int64_t ta = segment_nodes[2 * s], tb = segment_nodes[2 * s + 1];
-          const auto ia = lookup.find(ta), ib = lookup.find(tb);
-          if (ia == lookup.end() || ib == lookup.end())
-            throw std::runtime_error("line mesh: missing node");
-          const double xa = node_x[ia->second], xb = node_x[ib->second];
-          if (xa == xb)
-            throw std::runtime_error("line mesh: zero-length element " + std::to_string(s + 1));
-          const bool fwd = xa < xb;
-          c->lines.push_back(fwd ? LineElem{ta, tb, xa, xb} : LineElem{tb, ta, xb, xa});
-        }
-        if (c->lines.empty())
-          throw std::runtime_error("line mesh: document contains no line elements");
-        std::stable_sort(c->lines.begin(), c->lines.end(),
-                         [](const LineElem &a, const LineElem &b) { return a.xl < b.xl; });
-        c->lfaces.push_back({0, -1, c->lines.front().xl, -1.0});
-        for (size_t i = 0; i + 1 < c->lines.size(); ++i) {
-          if (c->lines[i].rn != c->lines[i + 1].ln)
-            throw std::runtime_error("line mesh: gap or discontinuity between elements " +
-                                     std::to_string(i) + " and " + std::to_string(i + 1));
-          c->lfaces.push_back({static_cast<int32_t>(i), static_cast<int32_t>(i + 1),
-                               c->lines[i].xr, 1.0});
-        }
-        c->lfaces.push_back(
-            {static_cast<int32_t>(c->lines.size() - 1), -1, c->lines.back().xr, 1.0});
+        build_lines(*c, node_count, node_tags, node_x, segment_count, segment_nodes);
+        build_stencil(*c);
+      }) != 0) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+// src/case/euler_line.cpp:14-48: line limiter stencil with periodic or open
+// ends (line_boundary_resolution.cpp pairs periodic ends).
+orc_case *orc_line_euler_create(int64_t node_count, const int64_t *node_tags,
+                                const double *node_x, int64_t segment_count,
+                                const int64_t *segment_nodes, int order, double gamma,
+                                int left_kind, int right_kind) {
+  orc_case *c = new orc_case;
+  if (guarded([&] {
+        if (order < 0 || order > 3)
+          throw std::runtime_error("oracle: polynomial order must be 0..3");
+        if ((left_kind == ORC_PERIODIC) != (right_kind == ORC_PERIODIC))
+          throw std::runtime_error("boundaries: periodic ends must be paired");
+        c->dim = 1;
+        c->p = order;
+        c->n = order + 1;
+        c->V = 4;
+        c->gamma = gamma;
+        c->left_kind = left_kind;
+        c->right_kind = right_kind;
+        build_lines(*c, node_count, node_tags, node_x, segment_count, segment_nodes);
         build_stencil(*c);
       }) != 0) {
     delete c;
@@ -1657,8 +1874,8 @@ int orc_residual(const orc_case *c, const double *state, double *rhs) {
 int orc_limit(const orc_case *c, double threshold, double density_floor, double pressure_floor,
               double *state, uint8_t *flags) {
   return guarded([&] {
-    if (c->dim != 2 || c->scalar)
-      throw std::runtime_error("oracle: the limiter is implemented for 2D Euler");
+    if (!c->is_euler())
+      throw std::runtime_error("oracle: the limiter applies to Euler cases");
     limit(*c, threshold, density_floor, pressure_floor, state, flags);
   });
 }

@@ -54,6 +54,9 @@ def load(kind: str = "port") -> C.CDLL:
     lib.orc_tri_euler_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int]
     lib.orc_tri_scalar_create.restype = _P
     lib.orc_tri_scalar_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, _d]
+    lib.orc_line_euler_create.restype = _P
+    lib.orc_line_euler_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int,
+                                          C.c_int]
     lib.orc_line_scalar_create.restype = _P
     lib.orc_line_scalar_create.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_int, _d, C.c_int,
                                            _d, C.c_int, _d]
@@ -322,6 +325,20 @@ def tri_scalar(node_tags, node_xy, triangles, order, velocity_x, velocity_y,
     if not h:
         raise OracleError(lib.orc_error().decode())
     return Case(lib, h, 2)
+
+
+def line_euler(node_tags, node_x, segments, order, gamma=1.4, left=PERIODIC, right=PERIODIC,
+               kind="port") -> Case:
+    """1D Euler on a line mesh, state U[variable][mode][element]."""
+    lib = load(kind)
+    tags = np.ascontiguousarray(node_tags, np.int64)
+    x = np.ascontiguousarray(node_x, np.float64)
+    seg = np.ascontiguousarray(segments, np.int64).reshape(-1)
+    h = lib.orc_line_euler_create(tags.size, tags, x, seg.size // 2, seg, order, gamma, left,
+                                  right)
+    if not h:
+        raise OracleError(lib.orc_error().decode())
+    return Case(lib, h, 1)
 
 
 def line_scalar(node_tags, node_x, segments, order, wavespeed, left=(PERIODIC, 0.0),

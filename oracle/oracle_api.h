@@ -64,6 +64,13 @@ orc_case *orc_tri_scalar_create(int64_t node_count, const int64_t *node_tags, co
                                 int64_t triangle_count, const int64_t *triangle_nodes, int order,
                                 double velocity_x, double velocity_y);
 
+/* 1D Euler on a line mesh (euler_line.cpp:14-48): periodic or outflow ends
+ * (ORC_INFLOW throws at the first residual, as line_euler_exterior does). */
+orc_case *orc_line_euler_create(int64_t node_count, const int64_t *node_tags,
+                                const double *node_x, int64_t segment_count,
+                                const int64_t *segment_nodes, int order, double gamma,
+                                int left_kind, int right_kind);
+
 /* 1D scalar advection on a line mesh. Segments list two node tags. */
 orc_case *orc_line_scalar_create(int64_t node_count, const int64_t *node_tags,
                                  const double *node_x, int64_t segment_count,
@@ -96,7 +103,8 @@ int orc_tri_trace(const orc_case *c, int32_t *donors, double *basis);
 
 /* Limiter stencil (euler_limiter_stencil_*.cpp): neighbors [E][K],
  * transforms [E][K][Np][Np], normals [E][2], check_basis [C][Np].
- * K = 3 (triangles) or 2 (lines); C from orc_stencil_check_count. */
+ * K = 3 (triangles) or 2 (lines); C from orc_stencil_check_count. Line
+ * scalar cases have no stencil (C = 0). */
 int64_t orc_stencil_check_count(const orc_case *c);
 int orc_stencil(const orc_case *c, int32_t *neighbors, double *transforms, double *normals,
                 double *check_basis);

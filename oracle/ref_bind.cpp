@@ -7,6 +7,7 @@
 // limiter, CFL, SSP-RK3, projection, diagnostics); this file only adapts
 // arrays to the reference types and composes the calls the way the
 // reference's case drivers do (src/case/euler_triangle.cpp:16-50,
+// src/case/euler_line.cpp:14-48,
 // src/case/scalar_advection_line.cpp:12-31,
 // src/case/scalar_advection_triangle.cpp:15-46, src/case/serial_case_runner.cpp).
 //
@@ -35,6 +36,8 @@
 #include "wasl/case/scalar_advection_case.hpp"
 #include "wasl/case/scalar_coefficient_layout.hpp"
 #include "wasl/case/scalar_diagnostics.hpp"
+#include "wasl/boundaries/line_euler_exterior.hpp"
+#include "wasl/dg/euler/line_residual.hpp"
 #include "wasl/dg/euler/triangle_residual.hpp"
 #include "wasl/dg/l2_projection.hpp"
 #include "wasl/dg/modal_layout.hpp"
@@ -80,6 +83,7 @@ template <typename F> int guarded(F &&f) {
 struct orc_case {
   int dim = 2;
   bool tri_scalar = false;  // 2D scalar advection, U[element][mode]
+  bool line_euler = false;  // 1D Euler, U[variable][mode][element]
   int p = 0;
   std::size_t n = 0;
   int boundary = ORC_PERIODIC;
@@ -96,7 +100,7 @@ struct orc_case {
 
   std::size_t E() const { return dim == 2 ? tri.num_elements() : line.num_elements(); }
   std::size_t F() const { return dim == 2 ? tri.faces.size() : line.faces.size(); }
-  std::size_t V() const { return dim == 2 && !tri_scalar ? 4 : 1; }
+  std::size_t V() const { return (dim == 2 && !tri_scalar) || line_euler ? 4 : 1; }
 };
 
 namespace {
@@ -128,7 +132,11 @@ std::vector<double> own_trace_exterior(const orc_case &c, const std::vector<doub
 
 void residual(const orc_case &c, const std::vector<double> &U, std::vector<double> &rhs) {
   rhs.resize(U.size());
-  if (c.tri_scalar) {
+  if (c.line_euler) {
+    const auto ext = wasl::line_euler_exterior(c.line, c.p, c.conditions, U);
+    wasl::euler_line_residual(c.line, c.equation, wasl::hllc_flux, c.p, U.data(), ext.data(),
+                              rhs.data());
+  } else if (c.tri_scalar) {
     const auto ext = wasl::periodic_triangle_exterior(c.trace, U, c.n);
     wasl::triangle_scalar_advection_residual(c.tri, c.advection, c.p, U.data(), ext.data(),
                                              rhs.data());
@@ -202,6 +210,26 @@ wasl::EulerPrimitive analytic_euler(int ic, const double *prm, double gamma, dou
 }
 
 void project(const orc_case &c, int ic, const double *prm, int64_t np, std::vector<double> &U) {
+  if (c.line_euler) {
+    if (ic != ORC_IC_EULER_REGIONS)
+      throw std::runtime_error("oracle: 1D Euler takes region initial states");
+    wasl::EulerCase config;
+    config.dimensions = 1;
+    config.polynomial_order = c.p;
+    for (int64_t k = 0; k + 10 <= np; k += 10) {
+      wasl::EulerRegion region;
+      region.shape = static_cast<wasl::EulerRegion::Shape>(static_cast<int>(prm[k]));
+      region.lower = prm[k + 1];
+      region.upper = prm[k + 2];
+      region.center_x = prm[k + 3];
+      region.center_y = prm[k + 4];
+      region.radius = prm[k + 5];
+      region.value = {prm[k + 6], prm[k + 7], prm[k + 8], prm[k + 9]};
+      config.initial_regions.push_back(region);
+    }
+    U = wasl::initial_coefficients(c.line, config, c.equation);
+    return;
+  }
   if (c.tri_scalar) {
     if (ic == ORC_IC_SCALAR_CIRCLES) {
       wasl::ScalarAdvectionCase config;
@@ -272,12 +300,16 @@ void project(const orc_case &c, int ic, const double *prm, int64_t np, std::vect
 }
 
 double mass(const orc_case &c, const std::vector<double> &U) {
+  if (c.line_euler)
+    return wasl::euler_density_mass(c.line, U);
   if (c.tri_scalar)
     return wasl::scalar_mass(c.tri, U, c.n);
   return c.dim == 2 ? wasl::euler_density_mass(c.tri, U) : wasl::scalar_mass(c.line, U, c.n);
 }
 
 double stable_dt(const orc_case &c, const std::vector<double> &U, double cfl) {
+  if (c.line_euler)
+    return wasl::euler_cfl_timestep(c.line, c.equation, U, c.n, c.p, cfl);
   if (c.tri_scalar)
     return wasl::triangle_scalar_cfl_timestep(c.tri, c.advection.velocity_x,
                                               c.advection.velocity_y, c.p, cfl);
@@ -287,7 +319,7 @@ double stable_dt(const orc_case &c, const std::vector<double> &U, double cfl) {
 
 void step(orc_case &c, double dt, int limiting, double thr, double rf, double pf,
           std::vector<double> &U) {
-  if (c.dim == 1) {
+  if (c.dim == 1 && !c.line_euler) {
     wasl::ssp_rk3_step(c.line, c.scalar, c.p, dt, c.exterior, U, c.workspace);
     return;
   }
@@ -358,6 +390,38 @@ orc_case *orc_tri_scalar_create(int64_t node_count, const int64_t *node_tags, co
             triangle_document(node_count, node_tags, node_xy, triangle_count, triangle_nodes));
         c->tri = wasl::triangle_mesh_from_prepared(wasl::prepare_mesh(built));
         c->trace = wasl::periodic_triangle_trace(c->tri, order);
+      }) != 0) {
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+orc_case *orc_line_euler_create(int64_t node_count, const int64_t *node_tags,
+                                const double *node_x, int64_t segment_count,
+                                const int64_t *segment_nodes, int order, double gamma,
+                                int left_kind, int right_kind) {
+  auto *c = new orc_case;
+  if (guarded([&] {
+        if ((left_kind == ORC_PERIODIC) != (right_kind == ORC_PERIODIC))
+          throw std::runtime_error("boundaries: periodic ends must be paired");
+        c->dim = 1;
+        c->line_euler = true;
+        c->p = order;
+        c->n = wasl::line_mode_count(order);
+        c->equation = wasl::Euler{gamma};
+        wasl::MshDocument doc;
+        for (int64_t i = 0; i < node_count; ++i)
+          doc.nodes.push_back({node_tags[i], 1, 1, node_x[i], 0.0, 0.0});
+        for (int64_t s = 0; s < segment_count; ++s)
+          doc.elements.push_back({s + 1, 1, 1, wasl::msh_line_element,
+                                  {segment_nodes[2 * s], segment_nodes[2 * s + 1]}});
+        c->line = wasl::line_mesh_from_prepared(wasl::prepare_mesh(wasl::build_line_mesh(doc)));
+        c->conditions.left = {static_cast<wasl::BoundaryKind>(left_kind), 0.0};
+        c->conditions.right = {static_cast<wasl::BoundaryKind>(right_kind), 0.0};
+        c->stencil = wasl::make_euler_limiter_stencil(
+            c->line, order,
+            left_kind == ORC_PERIODIC ? wasl::LineEnds::periodic : wasl::LineEnds::open);
       }) != 0) {
     delete c;
     return nullptr;
@@ -463,7 +527,7 @@ int orc_tri_trace(const orc_case *c, int32_t *donors, double *basis) {
 }
 
 int64_t orc_stencil_check_count(const orc_case *c) {
-  return c->dim == 2 && !c->tri_scalar ? static_cast<int64_t>(c->stencil.check_point_count()) : 0;
+  return c->V() == 4 ? static_cast<int64_t>(c->stencil.check_point_count()) : 0;
 }
 
 int orc_stencil(const orc_case *c, int32_t *neighbors, double *transforms, double *normals,
@@ -492,8 +556,8 @@ int orc_residual(const orc_case *c, const double *state, double *rhs) {
 int orc_limit(const orc_case *c, double threshold, double density_floor, double pressure_floor,
               double *state, uint8_t *flags) {
   return guarded([&] {
-    if (c->dim != 2 || c->tri_scalar)
-      throw std::runtime_error("oracle: the limiter is implemented for 2D Euler");
+    if (c->V() != 4)
+      throw std::runtime_error("oracle: the limiter applies to Euler cases");
     std::vector<double> U(state, state + 4 * c->n * c->E());
     if (flags)
       sensor_flags(*c, U, threshold, flags);

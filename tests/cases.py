@@ -26,6 +26,14 @@ JUMP = (0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.2, 1.0,
         2, 0, 0, 0.1, -0.2, 0.45, 0.2, 0.0, 0.0, 0.15)
 
 
+# 1D Euler regions (width 10): a default state and an interval [0.3, 0.6] of
+# low density and pressure, the case-regression test's limited shock tube
+SHOCK_TUBE = (0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.0, 1.0,
+              1, 0.3, 0.6, 0, 0, 0, 0.2, 0.0, 0.0, 0.15)
+# Sod (cases/tutorials/euler/sod-shock-tube/initial/state.toml)
+SOD = (0, 0, 0, 0, 0, 0, 1.0, 0.0, 0.0, 1.0,
+       1, 0.5, 1.0, 0, 0, 0, 0.125, 0.0, 0.0, 0.1)
+
 # 2D scalar advection: plane wave sin(2 pi (kx x + ky y) / L) on [0,10]^2 and
 # a disc of value 2 on a 0.25 background (circles: default, cx, cy, r, value)
 SCALAR_WAVE = (10.0, 1.0, 2.0)

@@ -1,8 +1,8 @@
 """Writes the golden fixtures under tests/golden/ (run in a container where
 the reference is present: `python tests/golden/make_fixtures.py`).
 
-  case_regression_rows.txt  the 22 on-path rows (euler-d2-*, scalar-d1-*,
-                            scalar-d2-*) of
+  case_regression_rows.txt  the 28 on-path rows (euler-d1-*, euler-d2-*,
+                            scalar-d1-*, scalar-d2-*), i.e. every row of
                             the reference's stored case-regression metrics
                             (cpp/tests/runtime/case_regression_reference.txt),
                             copied verbatim as data;
@@ -11,6 +11,7 @@ the reference is present: `python tests/golden/make_fixtures.py`).
                             on small seeded meshes: residuals P0-P3, limiter
                             flags and states, stepping trajectories, 1D
                             advection, 2D scalar advection residuals and
+                            steps, 1D Euler residuals, limiter and limited
                             steps.
 
 The reference cannot travel to the GPU box; these vectors do.
@@ -35,7 +36,7 @@ def regression_rows() -> None:
     keep = []
     with open(REFERENCE_TXT) as fh:
         for line in fh:
-            if line.startswith(("#", "euler-d2-", "scalar-d1-", "scalar-d2-")):
+            if line.startswith(("#", "euler-d1-", "euler-d2-", "scalar-d1-", "scalar-d2-")):
                 keep.append(line)
     with open(os.path.join(HERE, "case_regression_rows.txt"), "w") as fh:
         fh.writelines(keep)
@@ -109,6 +110,24 @@ def vectors() -> None:
         U = c.step(U, dt)
         traj.append(U)
     out["advect2d_p2_steps"] = np.stack(traj)
+
+    import refmeshes as RM
+
+    lt, lx, ls = RM.irregular_line(12)
+    for p in range(4):
+        for bc, tag in ((O.PERIODIC, "periodic"), (O.OUTFLOW, "outflow")):
+            c = O.line_euler(lt, lx, ls, p, left=bc, right=bc, kind="reference")
+            U = c.project(O.IC_EULER_REGIONS, K.SHOCK_TUBE)
+            Ul, flags = c.limit(U, 1e-3)
+            out[f"euler1d_p{p}_{tag}_state"] = U
+            out[f"euler1d_p{p}_{tag}_residual"] = c.residual(U)
+            out[f"euler1d_p{p}_{tag}_limited"] = Ul
+            out[f"euler1d_p{p}_{tag}_flags"] = flags
+            out[f"euler1d_p{p}_{tag}_dt"] = np.array([c.stable_dt(Ul, 0.2)])
+            traj = [Ul]
+            for _ in range(3):
+                traj.append(c.step(traj[-1], c.stable_dt(traj[-1], 0.2), True, 1e-3))
+            out[f"euler1d_p{p}_{tag}_steps"] = np.stack(traj)
     np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
 
 
