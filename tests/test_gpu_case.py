@@ -62,6 +62,15 @@ def test_coarse_calls_reproduce_case_regression_rows(gpu, tmp_path):
         got[name + "-restart"], _ = _measure(case, s)
     for p in range(3):
         for lim in (False, True):
+            name = f"euler-d1-p{p}" + ("-limited" if lim else "")
+            inside = (0.2, 0.0, 0.0, 0.15) if lim else (0.9, 0.25, 0.0, 0.95)
+            s = case.run_euler_case(
+                line_path, tmp_path / name, "", lim, 1e-4, 1e-10, 1e-10, 1, p, 1.4, 0.2, 0.0,
+                0.03, 1000, 2, 3, 4, 1,
+                regions=[0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.0, 1.0, 1, 0.3, 0.6, 0, 0, 0, *inside])
+            got[name], _ = _measure(case, s)
+    for p in range(3):
+        for lim in (False, True):
             name = f"euler-d2-p{p}" + ("-limited" if lim else "")
             inside = (0.2, 0.0, 0.0, 0.15) if lim else (0.9, 0.25, 0.1, 0.95)
             s = case.run_euler_case(

@@ -1,5 +1,5 @@
 """End-to-end runs through the HIP engine against the reference's own stored
-results: the 22 on-path case-regression rows (relative 1e-10, the reference's
+results: all 28 case-regression rows (relative 1e-10, the reference's
 tolerance, case_regression_test.cpp:40-44), vectors written by the reference's
 code (tests/golden/reference_vectors.npz) and the vortex convergence order."""
 
@@ -26,6 +26,10 @@ class _GpuCase:
             self.plain = wasl.Engine(wasl.Mesh.lines(*mesh), wasl.SCALAR_ADVECTION, p,
                                      wavespeed=0.7)
             self.limited = None
+        elif kind == "line_euler":
+            m = wasl.Mesh.lines(*mesh)
+            self.plain = wasl.Engine(m, wasl.EULER, p)
+            self.limited = wasl.Engine(m, wasl.EULER, p, limiting=True, sensor_threshold=1e-4)
         elif kind == "tri_scalar":
             self.plain = wasl.Engine(wasl.Mesh.triangles(*mesh), wasl.SCALAR_ADVECTION, p,
                                      wavespeed=0.7, velocity_y=-0.4)

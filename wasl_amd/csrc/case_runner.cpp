@@ -372,19 +372,24 @@ void validate_scalar_advection_case(const ScalarAdvectionCase &c) {
 }
 
 // src/case/euler_case.cpp:8-12 with run_triangle_euler
-// (src/case/euler_triangle.cpp:15-47).
+// (src/case/euler_triangle.cpp:15-47) and run_line_euler
+// (src/case/euler_line.cpp:14-48).
 SimulationSummary run_euler_case(const EulerCase &config) {
   validate_euler_case(config);
   const auto mesh = load_case_mesh(config.files.prepared_mesh, config.dimensions);
-  if (config.dimensions == 1)
-    throw std::runtime_error("wasl_hip: Euler on lines (run_line_euler) is not part of this engine");
-  int boundary = WASL_HIP_PERIODIC;
-  if (!all_of_kind(config.boundaries, BoundaryKind::periodic)) {
+  int boundary = WASL_HIP_PERIODIC, boundary_right = WASL_HIP_PERIODIC;
+  if (config.dimensions == 1) {
+    BoundaryCondition left, right;
+    resolve_line_boundaries(mesh.prepared, config.boundaries, left, right);
+    boundary = static_cast<int>(left.kind);
+    boundary_right = static_cast<int>(right.kind);
+  } else if (!all_of_kind(config.boundaries, BoundaryKind::periodic)) {
     // Beyond the reference: every boundary transmissive (own trace outside).
     if (!config.boundaries.empty() && all_of_kind(config.boundaries, BoundaryKind::outflow))
       boundary = WASL_HIP_OUTFLOW;
     else
       throw std::runtime_error("simulation: 2D Euler boundaries must be periodic");
+    boundary_right = boundary;
   }
   auto m = engine_mesh(mesh.prepared);
   wasl_hip_numerics num;
@@ -392,7 +397,8 @@ SimulationSummary run_euler_case(const EulerCase &config) {
   num.equation = WASL_HIP_EULER;
   num.polynomial_order = config.polynomial_order;
   num.heat_capacity_ratio = config.heat_capacity_ratio;
-  num.boundary_left = num.boundary_right = boundary;
+  num.boundary_left = boundary;
+  num.boundary_right = boundary_right;
   num.limiting = config.limiting ? 1 : 0;
   num.sensor_threshold = config.sensor_threshold;
   num.density_floor = config.density_floor;

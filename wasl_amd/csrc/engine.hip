@@ -238,6 +238,16 @@ struct wasl_hip_engine {
     return m;
   }
 
+  whip::EulerLineDev euler_line_mesh() const {
+    whip::EulerLineDev m;
+    m.E = E;
+    m.inv_jac = inv_jac.as<double>();
+    m.table = table.as<double>();
+    m.gamma = num.heat_capacity_ratio;
+    m.periodic = num.boundary_left == WASL_HIP_PERIODIC ? 1 : 0;
+    return m;
+  }
+
   whip::LineMeshDev line_mesh() const {
     whip::LineMeshDev m;
     m.E = E;
@@ -267,6 +277,11 @@ struct wasl_hip_engine {
   void build_stencil_host() {
     if (have_stencil || !euler())
       return;
+    if (dim == 1) {
+      stencil = whip::line_limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC);
+      have_stencil = true;
+      return;
+    }
     stencil = whip::limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC
                                                  ? trace.donors
                                                  : std::vector<int32_t>{},
@@ -442,6 +457,26 @@ std::vector<float> mixed_tables(int p) {
   }
 }
 
+// 1D Euler (whip::LineE<P>): basis and w_q dphi/dxi at the p+1 Gauss points,
+// then phi(-1), phi(+1).
+std::vector<double> euler_line_tables(int p) {
+  const int Np = p + 1, NQ = p + 1;
+  std::vector<double> t(2 * NQ * Np + 2 * Np, 0.0);
+  const whip::LineRule g = whip::gauss_legendre(NQ);
+  std::vector<double> b(Np), d(Np);
+  for (int q = 0; q < NQ; ++q) {
+    whip::legendre(g.x[q], p, b.data());
+    whip::legendre_gradient(g.x[q], p, d.data());
+    for (int m = 0; m < Np; ++m) {
+      t[q * Np + m] = b[m];
+      t[NQ * Np + q * Np + m] = g.w[q] * d[m];
+    }
+  }
+  whip::legendre(-1.0, p, t.data() + 2 * NQ * Np);
+  whip::legendre(1.0, p, t.data() + 2 * NQ * Np + Np);
+  return t;
+}
+
 std::vector<double> line_tables(int p) {
   const int Np = p + 1, NQ = p > 0 ? p : 1;
   std::vector<double> t(2 * NQ * Np + 2 * Np, 0.0);
@@ -502,6 +537,12 @@ template <int P> void launch_advect_p(const wasl_hip_engine &g, const whip::Stag
   hipLaunchKernelGGL((whip::advect_stage<P>), grid, dim3(256), 0, g.stream, g.advect_mesh(), a);
 }
 
+template <int P> void launch_euler_line_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
+  const dim3 grid((g.E + 255) / 256);
+  hipLaunchKernelGGL((whip::euler_line_stage<P>), grid, dim3(256), 0, g.stream,
+                     g.euler_line_mesh(), a);
+}
+
 template <int P> void launch_line_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
   const dim3 grid((g.E + 255) / 256);
   hipLaunchKernelGGL((whip::line_stage<P>), grid, dim3(256), 0, g.stream, g.line_mesh(), a);
@@ -521,6 +562,13 @@ void launch_stage(const wasl_hip_engine &g, const whip::StageArgs &a) {
     case 1: launch_euler_p<1>(g, a); break;
     case 2: launch_euler_p<2>(g, a); break;
     default: launch_euler_p<3>(g, a); break;
+    }
+  } else if (g.euler()) {
+    switch (g.p) {
+    case 0: launch_euler_line_p<0>(g, a); break;
+    case 1: launch_euler_line_p<1>(g, a); break;
+    case 2: launch_euler_line_p<2>(g, a); break;
+    default: launch_euler_line_p<3>(g, a); break;
     }
   } else {
     switch (g.p) {
@@ -559,6 +607,14 @@ void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip:
                     int parity) {
   const whip::LimiterDev L = g.limiter();
   const dim3 grid(whip::limit_blocks(g.E)), grid256((g.E + 255) / 256);
+  if (g.dim == 1) {
+    hipLaunchKernelGGL((whip::limit_troubled_line<P>), grid, dim3(64), 0, g.stream, L, U,
+                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity,
+                       g.err.as<int>());
+    hipLaunchKernelGGL((whip::scatter_limited_line<P>), grid256, dim3(256), 0, g.stream, U, g.E,
+                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity);
+    return;
+  }
   hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, g.stream, L, U,
                      g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.err.as<int>());
   hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, g.stream, U, g.E,
@@ -579,7 +635,9 @@ void launch_limit(const wasl_hip_engine &g, double *U, int slot, whip::RunCtl *c
 }
 
 template <int P> void launch_sensor_p(const wasl_hip_engine &g, uint8_t *flags) {
-  hipLaunchKernelGGL((whip::sensor<P>), dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
+  constexpr int kTriNp = (P + 1) * (P + 2) / 2;
+  const auto kernel = g.dim == 1 ? whip::sensor<P + 1> : whip::sensor<kTriNp>;
+  hipLaunchKernelGGL(kernel, dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
                      g.u.as<double>(), g.E, g.num.sensor_threshold, g.stencil.highest_begin,
                      flags, g.list.as<int>(), g.counters.as<int>() + 3);
 }
@@ -663,7 +721,7 @@ void raise_device_error(wasl_hip_engine &g) {
 double mass_of(wasl_hip_engine &g) {
   const int blocks = (g.E + 255) / 256;
   hipLaunchKernelGGL(whip::mass_partials, dim3(blocks), dim3(256), 0, g.stream,
-                     g.u.as<double>(), g.dim == 2 ? 1 : g.Np, g.mass_w.as<double>(), g.E,
+                     g.u.as<double>(), g.euler() || g.dim == 2 ? 1 : g.Np, g.mass_w.as<double>(), g.E,
                      g.partials.as<double>());
   hipLaunchKernelGGL(whip::mass_finish, dim3(1), dim3(64), 0, g.stream, g.partials.as<double>(),
                      blocks, g.result.as<double>());
@@ -688,7 +746,7 @@ double resident_speed(wasl_hip_engine &g) {
   HIPCHECK(hipMemcpyAsync(g.ctl.p, h, sizeof(*h), hipMemcpyHostToDevice, g.stream));
   hipLaunchKernelGGL(whip::cfl_speed, dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
                      g.u.as<double>(), g.E, g.Np, g.num.heat_capacity_ratio,
-                     g.ctl.as<whip::RunCtl>(), 0);
+                     g.ctl.as<whip::RunCtl>(), 0, g.dim);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(h, g.ctl.p, sizeof(*h), hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(hipStreamSynchronize(g.stream));
@@ -871,9 +929,58 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.result_host), sizeof(double)));
   size_t &tot = g.device_bytes;
 
-  if (num.equation == WASL_HIP_EULER) {
+  if (num.equation == WASL_HIP_EULER && mesh.dimension == 1) {
+    // run_line_euler (src/case/euler_line.cpp:14-48): periodic or outflow
+    // ends, the line limiter stencil, FP64.
+    const int kinds[2] = {num.boundary_left, num.boundary_right};
+    for (int k : kinds) {
+      if (k < 0 || k > 2)
+        throw std::runtime_error("boundaries: invalid boundary kind");
+      if (k == WASL_HIP_INFLOW)
+        throw std::runtime_error("line Euler exterior: inflow is not implemented");
+    }
+    if ((num.boundary_left == WASL_HIP_PERIODIC) != (num.boundary_right == WASL_HIP_PERIODIC))
+      throw std::runtime_error("boundaries: periodic ends must be paired");
+    if (num.precision != WASL_HIP_FP64)
+      throw std::runtime_error("engine: 1D Euler runs in FP64");
+    g.dim = 1;
+    g.V = 4;
+    g.Np = g.p + 1;
+    g.E = static_cast<int>(mesh.lines.size());
+    g.F = static_cast<int>(mesh.lfaces.size());
+    g.lanes = 1;
+    g.stage_kernel = 1;
+    std::vector<double> ij(g.E), mw(g.E);
+    for (int e = 0; e < g.E; ++e) {
+      const double jac = (mesh.lines[e].xr - mesh.lines[e].xl) / 2.0;
+      ij[e] = 1.0 / jac;
+      mw[e] = std::sqrt(2.0) * jac;
+    }
+    upload(g.inv_jac, ij, tot, g.stream);
+    upload(g.mass_w, mw, tot, g.stream);
+    upload(g.table, euler_line_tables(g.p), tot, g.stream);
+    g.hmin = whip::minimum_length(mesh);
+    const size_t E = static_cast<size_t>(g.E);
+    if (num.limiting && g.p > 0) {
+      g.build_stencil_host();
+      upload(g.lnbr, g.stencil.neighbors, tot, g.stream);
+      upload(g.T, g.stencil.transforms, tot, g.stream);
+      std::vector<double> nrm(2 * E);
+      for (size_t e = 0; e < E; ++e) {
+        nrm[2 * e] = g.stencil.nx[e];
+        nrm[2 * e + 1] = g.stencil.ny[e];
+      }
+      upload(g.nrm, nrm, tot, g.stream);
+      upload(g.check, g.stencil.check, tot, g.stream);
+      g.side.alloc(E * 4 * g.Np * sizeof(double), tot);
+    } else {
+      g.stencil.highest_begin = g.p > 0 ? g.Np - 1 : g.Np;
+    }
+    g.flags.alloc(E, tot);
+    g.list.alloc(E * sizeof(int), tot);
+  } else if (num.equation == WASL_HIP_EULER) {
     if (mesh.dimension != 2)
-      throw std::runtime_error("engine: Euler runs on triangle meshes");
+      throw std::runtime_error("engine: Euler runs on line or triangle meshes");
     if (num.boundary_left != WASL_HIP_PERIODIC && num.boundary_left != WASL_HIP_OUTFLOW)
       throw std::runtime_error("simulation: 2D Euler boundaries must be periodic or outflow");
     g.dim = 2;
@@ -1051,7 +1158,7 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
   g.result.alloc(sizeof(double), tot);
   HIPCHECK(hipStreamSynchronize(g.stream));
-  if (g.euler() && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
+  if (g.euler() && g.dim == 2 && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
       num.lanes_per_element == 0)
     autotune(g);
 }
@@ -1519,7 +1626,8 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     whip::RunCtl *dctl = g->ctl.as<whip::RunCtl>();
     if (g->euler())
       hipLaunchKernelGGL(whip::cfl_speed, dim3((g->E + 255) / 256), dim3(256), 0, g->stream,
-                         g->u.as<double>(), g->E, g->Np, g->num.heat_capacity_ratio, dctl, 0);
+                         g->u.as<double>(), g->E, g->Np, g->num.heat_capacity_ratio, dctl, 0,
+                         g->dim);
     if (g->mixed())
       launch_traces(*g, g->u.as<double>(), g->tr_u.as<float>(), g->mxtable.as<float>(),
                     g->stream);

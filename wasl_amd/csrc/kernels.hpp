@@ -2030,12 +2030,178 @@ void advect_stage(AdvectMesh mesh, StageArgs a) {
   advect_body<P>(mesh, a, xcd_block(), blockIdx.x == 0, tab);
 }
 
+// --------------------------------------------------- 1D Euler stage ----
+
+// Tables of the line Euler stage (doubles): basis and weighted derivative at
+// the max(1, p+1) = p+1 Gauss points of the reference's volume rule
+// (dg/euler/line_residual.hpp:29), and the end values phi(-1), phi(+1).
+template <int P> struct LineE {
+  static constexpr int Np = P + 1;
+  static constexpr int NQ = P + 1;
+  static constexpr int kB = 0;                 // [NQ][Np]
+  static constexpr int kW = kB + NQ * Np;      // [NQ][Np] w_q dphi/dxi
+  static constexpr int kL = kW + NQ * Np;      // [Np]
+  static constexpr int kR = kL + Np;           // [Np]
+  static constexpr int kTable = kR + Np;
+};
+
+struct EulerLineDev {
+  int E;
+  const double *inv_jac;      // [E] 1 / (h/2)
+  const double *table;        // LineE<P>::kTable doubles
+  double gamma;
+  int periodic;               // 1: periodic ends; 0: outflow (own trace outside)
+};
+
+__device__ __forceinline__ C4 flux_x_of(double g1, const C4 &s) {
+  const double inv = rcp(s.r);
+  const double u = s.mx * inv, w = s.my * inv;
+  const double p = g1 * (s.E - 0.5 * (s.mx * u + s.my * w));
+  return {s.mx, s.mx * u + p, s.my * u, (s.E + p) * u};
+}
+
+// One lane per element, state U[4][Np][E] (euler_line_residual,
+// dg/euler/line_residual.hpp:23-86, with line_euler_exterior,
+// src/boundaries/line_euler_exterior.cpp:13-43). The elements run left to
+// right, so the faces of element e are shared with e - 1 and e + 1; every
+// face's HLLC flux is evaluated as (left, right, +x) from the same traces on
+// both sides. Epilogue as in the triangle stage: RK combination, the density
+// sensor and the CFL speed |u_x| + c of the new cell average.
+template <int P>
+__device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const StageArgs a,
+                                                int block, bool leader, double *tab, double *red) {
+  using D = LineE<P>;
+  constexpr int Np = D::Np, NQ = D::NQ;
+  const int E = mesh.E;
+  const int e_raw = block * 256 + static_cast<int>(threadIdx.x);
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+  const double g = mesh.gamma, g1 = g - 1.0;
+  double c[4][Np];
+  load_coefficients<Np>(a.uin, E, e, c);
+  double dt = a.dt;
+  if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
+    return;
+  if (a.sensor && leader && threadIdx.x == 0)
+    a.counters[(a.stage + 1) % 3] = 0;
+  for (int i = threadIdx.x; i < D::kTable; i += blockDim.x)
+    tab[i] = mesh.table[i];
+  __syncthreads();
+  const int en = e > 0 ? e - 1 : (mesh.periodic ? E - 1 : e);
+  const int ep = e < E - 1 ? e + 1 : (mesh.periodic ? 0 : e);
+  double cl[4][Np], cr[4][Np];
+  load_coefficients<Np>(a.uin, E, en, cl);
+  load_coefficients<Np>(a.uin, E, ep, cr);
+
+  double acc[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      acc[v][m] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const C4 f = flux_x_of(g1, trace_of<Np>(c, tab + D::kB + q * Np));
+    const double fv[4] = {f.r, f.mx, f.my, f.E};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        acc[v][m] += fv[v] * tab[D::kW + q * Np + m];
+  }
+  const double *bl = tab + D::kL, *br = tab + D::kR;
+  const C4 own_l = trace_of<Np>(c, bl), own_r = trace_of<Np>(c, br);
+  // outflow ends: the exterior is the owner's own trace at that end
+  const bool open_l = e == 0 && !mesh.periodic, open_r = e == E - 1 && !mesh.periodic;
+  const C4 ext_l = open_l ? own_l : trace_of<Np>(cl, br);
+  const C4 ext_r = open_r ? own_r : trace_of<Np>(cr, bl);
+  const C4 fl = hllc(g, ext_l, own_l, 1.0, 0.0);
+  const C4 fr = hllc(g, own_r, ext_r, 1.0, 0.0);
+  const double vl[4] = {fl.r, fl.mx, fl.my, fl.E}, vr[4] = {fr.r, fr.mx, fr.my, fr.E};
+  const double ij = mesh.inv_jac[e];
+  double out0[4], dens[Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      const double r = (acc[v][m] + bl[m] * vl[v] - br[m] * vr[v]) * ij;
+      const int idx = (v * Np + m) * E + e;
+      double y;
+      if (a.residual_only) {
+        y = r;
+      } else {
+        y = a.cb * (c[v][m] + dt * r);
+        if (a.ca != 0.0)
+          y += a.ca * a.un[idx];
+      }
+      if (valid)
+        a.out[idx] = y;
+      if (m == 0)
+        out0[v] = y;
+      if (v == 0)
+        dens[m] = y;
+    }
+  if (a.sensor && !a.residual_only && valid) {
+    // modal-decay sensor on density (euler_modal_limiter.cpp:18-24)
+    double total = 0.0, high = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      total += dens[m] * dens[m];
+      if (m >= a.highest_begin)
+        high += dens[m] * dens[m];
+    }
+    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
+    if (a.flags)
+      a.flags[e] = troubled ? 1 : 0;
+    if (troubled) {
+      const int slot = atomicAdd(&a.counters[a.stage], 1);
+      if (slot < E)
+        a.list[slot] = e;
+    }
+  }
+  if (a.want_lambda && !a.residual_only) {
+    // |u_x| + c of the new cell average (cfl_timestep.cpp:33-47, 1D)
+    double speed = 0.0;
+    if (valid) {
+      const double is2 = 1.0 / sqrt(2.0);
+      const double r = is2 * out0[0], mx = is2 * out0[1], my = is2 * out0[2], en2 = is2 * out0[3];
+      const double inv = 1.0 / r;
+      const double u = mx * inv, w = my * inv;
+      const double p = g1 * (en2 - 0.5 * (mx * u + my * w));
+      if (!(r > 0.0 && p > 0.0)) {
+        if (a.ctl)
+          a.ctl->s[a.parity ^ 1].bad = 1;
+      } else {
+        speed = fabs(u) + sqrt(g * p / r);
+      }
+    }
+    speed = wave_max(speed);
+    if ((threadIdx.x & 63) == 0)
+      red[threadIdx.x >> 6] = speed;
+    __syncthreads();
+    if (threadIdx.x == 0 && a.ctl) {
+      const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      atomicMax(&a.ctl->s[a.parity ^ 1].lambda,
+                static_cast<unsigned long long>(__double_as_longlong(mx)));
+    }
+  }
+  stage_epilogue(a.ctl, a.parity, a.stage, dt, leader);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void euler_line_stage(EulerLineDev mesh, StageArgs a) {
+  __shared__ double tab[LineE<P>::kTable];
+  __shared__ double red[4];
+  euler_line_body<P>(mesh, a, blockIdx.x, blockIdx.x == 0, tab, red);
+}
+
 // ------------------------------------------------------- reductions ----
 
 // CFL speed of the cell averages of U[4][Np][E] into ctl->s[parity].lambda
 // (cfl_timestep.cpp:33-47); an unphysical average sets the slot's bad flag.
+// dims 1: |u_x| + c (lines); dims 2: |u| + c (triangles).
 __global__ __launch_bounds__(256) void cfl_speed(const double *U, int E, int Np, double gamma,
-                                                 RunCtl *ctl, int parity) {
+                                                 RunCtl *ctl, int parity, int dims) {
   __shared__ double red[4];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double speed = 0.0;
@@ -2049,7 +2215,7 @@ __global__ __launch_bounds__(256) void cfl_speed(const double *U, int E, int Np,
     if (!(r > 0.0 && p > 0.0))
       ctl->s[parity].bad = 1;
     else
-      speed = hypot(u, w) + sqrt(gamma * p / r);
+      speed = (dims == 1 ? fabs(u) : hypot(u, w)) + sqrt(gamma * p / r);
   }
   speed = wave_max(speed);
   if ((threadIdx.x & 63) == 0)
@@ -2109,19 +2275,19 @@ __global__ void mass_finish(const double *partials, int blocks, double *result) 
 
 struct LimiterDev {
   int E;
-  const int *nbr;        // [E][3]
-  const double *T;       // [E][3][Np][Np]
+  const int *nbr;        // [E][K]: K = 3 (triangles) or 2 (lines)
+  const double *T;       // [E][K][Np][Np]
   const double *nrm;     // [E][2]
   const double *check;   // [NC][Np]
   double gamma, density_floor, pressure_floor;
 };
 
-// Standalone troubled-cell sensor over density modes.
-template <int P>
+// Standalone troubled-cell sensor over the Np density modes of an element
+// (triangles or lines; one mode means P0, which is never troubled).
+template <int Np>
 __global__ __launch_bounds__(256) void sensor(const double *U, int E, double threshold,
                                               int highest_begin, uint8_t *flags, int *list,
                                               int *count) {
-  constexpr int Np = Tri<P>::Np;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E)
     return;
@@ -2133,7 +2299,7 @@ __global__ __launch_bounds__(256) void sensor(const double *U, int E, double thr
     if (m >= highest_begin)
       high += v * v;
   }
-  const bool troubled = P > 0 && !(total == 0.0 || high / total <= threshold);
+  const bool troubled = Np > 1 && !(total == 0.0 || high / total <= threshold);
   if (flags)
     flags[e] = troubled ? 1 : 0;
   if (troubled)
@@ -2213,14 +2379,14 @@ __device__ __forceinline__ double wave_min(double x) {
 // modes, the candidates and the 64-step bisections of the check points
 // proceed side by side. Blocks of 64 threads take list entries first,
 // first + stride, ...
-template <int P>
-__device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double *U, double *side,
+template <int Np, int NC, int KMAX>
+__device__ __forceinline__ void limit_wave_core(const LimiterDev L, const double *U, double *side,
                                                 const int *list, const int *count, RunCtl *ctl,
                                                 int parity, int *err, int first, int stride) {
-  constexpr int Np = Tri<P>::Np, NC = Tri<P>::NC, NV = 4 * Np;
+  constexpr int NV = 4 * Np;
   static_assert(NV <= 64 && NC <= 64, "one wave per element");
-  __shared__ double t[4][Np], tc[4][Np], cv[4][Np], cc[3][4][Np], lim[4][Np], o[4][Np];
-  __shared__ double w0[4], wk[3][4], wsum[4];
+  __shared__ double t[4][Np], tc[4][Np], cv[4][Np], cc[KMAX][4][Np], lim[4][Np], o[4][Np];
+  __shared__ double w0[4], wk[KMAX][4], wsum[4];
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
   const int n = *count;
@@ -2243,8 +2409,8 @@ __device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double
       tc[v][m] = pick4(w, v);
     }
     int K = 0;
-    for (int k = 0; k < 3; ++k)
-      if (L.nbr[3 * e + k] >= 0)
+    for (int k = 0; k < KMAX; ++k)
+      if (L.nbr[KMAX * e + k] >= 0)
         ++K;
     __syncthreads();
     if (K == 0) {
@@ -2253,11 +2419,11 @@ __device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double
     } else {
       // candidate j in characteristic variables: T_j U_nb with the own mean
       int j = 0;
-      for (int k = 0; k < 3; ++k) {
-        const int nb = L.nbr[3 * e + k];
+      for (int k = 0; k < KMAX; ++k) {
+        const int nb = L.nbr[KMAX * e + k];
         if (nb < 0)
           continue;
-        const double *M = L.T + (static_cast<size_t>(e) * 3 + k) * Np * Np;
+        const double *M = L.T + (static_cast<size_t>(e) * KMAX + k) * Np * Np;
         if (vm) {
           double x = 0.0;
 #pragma unroll
@@ -2376,17 +2542,23 @@ __device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double
   }
 }
 
+template <int P>
+__device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double *U, double *side,
+                                                const int *list, const int *count, RunCtl *ctl,
+                                                int parity, int *err, int first, int stride) {
+  limit_wave_core<Tri<P>::Np, Tri<P>::NC, 3>(L, U, side, list, count, ctl, parity, err, first,
+                                             stride);
+}
+
 // Writes the limited coefficients back; with a trace buffer (mixed
 // precision) also the FP32 edge traces of the limited element.
-template <int P>
-__device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
-                                             const int *count, RunCtl *ctl, int parity, int idx,
-                                             float *tr = nullptr, const float *mxtab = nullptr) {
-  constexpr int Np = Tri<P>::Np;
+template <int Np>
+__device__ __forceinline__ bool scatter_core(double *U, int E, const double *side, const int *list,
+                                             const int *count, RunCtl *ctl, int parity, int idx) {
   if (ctl && (ctl->err || !ctl->s[parity].active))
-    return;
+    return false;
   if (idx >= *count)
-    return;
+    return false;
   const int e = list[idx];
   const double *src = side + static_cast<size_t>(idx) * 4 * Np;
 #pragma unroll
@@ -2394,6 +2566,18 @@ __device__ __forceinline__ void scatter_body(double *U, int E, const double *sid
 #pragma unroll
     for (int m = 0; m < Np; ++m)
       U[(v * Np + m) * E + e] = src[v * Np + m];
+  return true;
+}
+
+template <int P>
+__device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
+                                             const int *count, RunCtl *ctl, int parity, int idx,
+                                             float *tr = nullptr, const float *mxtab = nullptr) {
+  constexpr int Np = Tri<P>::Np;
+  if (!scatter_core<Np>(U, E, side, list, count, ctl, parity, idx))
+    return;
+  const int e = list[idx];
+  const double *src = side + static_cast<size_t>(idx) * 4 * Np;
   if (tr) {
     using X = Mx<P>;
     constexpr int NE = X::NE;
@@ -2428,6 +2612,24 @@ __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const d
                                                        const float *mxtab) {
   scatter_body<P>(U, E, side, list, count, ctl, parity, blockIdx.x * blockDim.x + threadIdx.x,
                   tr, mxtab);
+}
+
+// Lines: two neighbours, (P+1) Gauss points and the two ends as check points.
+template <int P>
+__global__ __launch_bounds__(64) void limit_troubled_line(LimiterDev L, const double *U,
+                                                          double *side, const int *list,
+                                                          const int *count, RunCtl *ctl,
+                                                          int parity, int *err) {
+  limit_wave_core<P + 1, P + 3, 2>(L, U, side, list, count, ctl, parity, err, blockIdx.x,
+                                   gridDim.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void scatter_limited_line(double *U, int E, const double *side,
+                                                            const int *list, const int *count,
+                                                            RunCtl *ctl, int parity) {
+  scatter_core<P + 1>(U, E, side, list, count, ctl, parity,
+                      blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Batched limiter: tiles of 64 troubled-list slots per (case, chunk); the

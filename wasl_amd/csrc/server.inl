@@ -236,7 +236,7 @@ void admit_ctl(wasl_hip_server &s, Request &r) {
   if (g.dim == 2)
     hipLaunchKernelGGL(whip::cfl_speed, dim3((g.E + 255) / 256), dim3(256), 0, s.stream,
                        g.u.as<double>(), g.E, g.Np, g.num.heat_capacity_ratio,
-                       g.ctl.as<whip::RunCtl>(), 0);
+                       g.ctl.as<whip::RunCtl>(), 0, 2);
   if (g.mixed())
     launch_traces(g, g.u.as<double>(), g.tr_u.as<float>(), s.mxtab[g.p].as<float>(), s.stream);
   HIPCHECK(hipGetLastError());
@@ -619,6 +619,11 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
       throw std::runtime_error("server: null argument");
     if (!(cfl > 0.0) || (end_steps == 0 && !(end_time > 0.0)) || max_steps == 0)
       throw std::runtime_error("server: invalid run control");
+    const bool euler = numerics->equation == WASL_HIP_EULER;
+    if (euler != (mesh->m.dimension == 2))
+      throw std::runtime_error(
+          "server: serving takes 2D Euler and 1D scalar advection cases; run 1D Euler and 2D "
+          "advection on their own engines");
     auto r = std::make_unique<Request>();
     r->mesh = mesh->m;
     r->num = *numerics;

@@ -561,6 +561,83 @@ PeriodicTrace periodic_trace(const Mesh &m, int p) {
 
 // ---------------------------------------------------------------- stencil ----
 
+LimiterStencil line_limiter_stencil(const Mesh &m, int p, bool periodic) {
+  LimiterStencil s;
+  const size_t E = m.lines.size();
+  const int Np = p + 1;
+  s.K = 2;
+  s.Np = Np;
+  s.highest_begin = p > 0 ? Np - 1 : Np;
+  s.neighbors.assign(E * 2, -1);
+  s.nx.assign(E, 1.0);
+  s.ny.assign(E, 0.0);
+  const auto add = [&](size_t e, int32_t n) {
+    int32_t *v = s.neighbors.data() + 2 * e;
+    for (int i = 0; i < 2; ++i) {
+      if (v[i] == n)
+        return;
+      if (v[i] < 0) {
+        v[i] = n;
+        return;
+      }
+    }
+  };
+  int32_t left_end = -1, right_end = -1;
+  for (const LineFace &f : m.lfaces) {
+    if (f.right >= 0) {
+      add(static_cast<size_t>(f.left), f.right);
+      add(static_cast<size_t>(f.right), f.left);
+    } else if (f.normal < 0.0) {
+      left_end = f.left;
+    } else {
+      right_end = f.left;
+    }
+  }
+  if (periodic && left_end >= 0 && right_end >= 0) {
+    add(static_cast<size_t>(left_end), right_end);
+    add(static_cast<size_t>(right_end), left_end);
+  }
+  // entry [m][s] += w_q phi_m(xi_q) phi_s(source xi of q), the neighbour moved
+  // by the period when its centre is more than half a period away
+  const LineRule rule = gauss_legendre(std::max(1, p + 1));
+  const double period = m.lines.back().xr - m.lines.front().xl;
+  s.transforms.assign(E * 2 * Np * Np, 0.0);
+  std::vector<double> sb(Np), tb(Np);
+  for (size_t e = 0; e < E; ++e) {
+    const double tc = (m.lines[e].xl + m.lines[e].xr) / 2.0;
+    const double tj = (m.lines[e].xr - m.lines[e].xl) / 2.0;
+    for (int k = 0; k < 2; ++k) {
+      const int32_t nb = s.neighbors[2 * e + k];
+      if (nb < 0)
+        continue;
+      double sl = m.lines[static_cast<size_t>(nb)].xl, sr = m.lines[static_cast<size_t>(nb)].xr;
+      const double gap = tc - (sl + sr) / 2.0;
+      const double shift = gap > 0.5 * period ? period : gap < -0.5 * period ? -period : 0.0;
+      sl += shift;
+      sr += shift;
+      const double sc = (sl + sr) / 2.0, sj = (sr - sl) / 2.0;
+      double *M = s.transforms.data() + (2 * e + k) * Np * Np;
+      for (int q = 0; q < rule.size(); ++q) {
+        legendre((tc + tj * rule.x[q] - sc) / sj, p, sb.data());
+        legendre(rule.x[q], p, tb.data());
+        for (int a = 0; a < Np; ++a)
+          for (int b = 0; b < Np; ++b)
+            M[a * Np + b] += rule.w[q] * tb[a] * sb[b];
+      }
+    }
+  }
+  std::vector<double> b(Np);
+  const auto append = [&](double xi) {
+    legendre(xi, p, b.data());
+    s.check.insert(s.check.end(), b.begin(), b.end());
+  };
+  for (int q = 0; q < rule.size(); ++q)
+    append(rule.x[q]);
+  append(-1.0);
+  append(1.0);
+  return s;
+}
+
 LimiterStencil limiter_stencil(const Mesh &m, int p, const std::vector<int32_t> &donors,
                                int points) {
   LimiterStencil s;
@@ -857,6 +934,27 @@ std::vector<double> project(const Mesh &m, int p, double gamma, int ic, const do
   std::vector<double> basis(static_cast<size_t>(rule.size()) * Np);
   for (int q = 0; q < rule.size(); ++q)
     legendre(rule.x[q], p, basis.data() + q * Np);
+  if (ic < IC_SCALAR_PULSE) {
+    // project_line_field of the conserved state, U[4][Np][E]
+    // (initial_state.cpp:48-54; euler_initial_value with y = 0)
+    std::vector<double> U(4 * Np * E, 0.0);
+    euler_value(ic, prm, np, gamma, 0.0, 0.0);
+    parallel_for(E, [&](size_t begin, size_t end) {
+      for (size_t e = begin; e < end; ++e) {
+        const double xl = m.lines[e].xl, xr = m.lines[e].xr;
+        for (int q = 0; q < rule.size(); ++q) {
+          const double x = (xl + xr) / 2.0 + (xr - xl) / 2.0 * rule.x[q];
+          const Prim w = euler_value(ic, prm, np, gamma, x, 0.0);
+          const double ke = 0.5 * w.r * (w.u * w.u + w.v * w.v);
+          const double s[4] = {w.r, w.r * w.u, w.r * w.v, w.p / (gamma - 1.0) + ke};
+          for (int v = 0; v < 4; ++v)
+            for (int k = 0; k < Np; ++k)
+              U[(static_cast<size_t>(v) * Np + k) * E + e] += rule.w[q] * s[v] * basis[q * Np + k];
+        }
+      }
+    });
+    return U;
+  }
   scalar_value(ic, prm, np, 0.0);
   std::vector<double> U(Np * E, 0.0);
   parallel_for(E, [&](size_t begin, size_t end) {

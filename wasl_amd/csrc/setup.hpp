@@ -129,6 +129,12 @@ struct LimiterStencil {
 LimiterStencil limiter_stencil(const Mesh &mesh, int p, const std::vector<int32_t> &donors,
                                int points);
 
+// Line limiter stencil (euler_limiter_stencil_line.cpp:14-94): K = 2
+// neighbours from the faces in order (the two end elements joined when the
+// ends are periodic), normals (1, 0), transforms by the Gauss rule of
+// max(1, p+1) points, check points at those Gauss points and the two ends.
+LimiterStencil line_limiter_stencil(const Mesh &mesh, int p, bool periodic);
+
 // Smallest altitude 4 detJ / longest edge (triangles) or smallest length.
 double minimum_length(const Mesh &mesh);
 
@@ -137,7 +143,7 @@ double minimum_length(const Mesh &mesh);
 enum { IC_EULER_REGIONS = 0, IC_VORTEX = 1, IC_RAREFACTION = 2, IC_SMOOTH = 3,
        IC_SCALAR_PULSE = 10, IC_SCALAR_REGIONS = 11, IC_SCALAR_CIRCLES = 12,
        IC_SCALAR_WAVE = 13 };
-// Euler (codes 0-3): U[4][Np][E]; scalar (10-13): U[E][Np].
+// Euler (codes 0-3): U[4][Np][E] (lines: x with y = 0); scalar (10-13): U[E][Np].
 std::vector<double> project(const Mesh &mesh, int p, double gamma, int ic, const double *params,
                             int64_t param_count);
 

@@ -245,11 +245,11 @@ class Mesh:
         count = prm.size
         if count == 0:
             prm = np.zeros(1)
+        V = 1 if ic >= IC_SCALAR_PULSE else 4
         if self.dimension == 2:
-            V = 1 if ic >= IC_SCALAR_PULSE else 4
             n = V * (order + 1) * (order + 2) // 2 * self.elements
         else:
-            n = (order + 1) * self.elements
+            n = V * (order + 1) * self.elements
         out = np.zeros(n)
         _check(library().wasl_hip_mesh_project(self.h, order, gamma, ic, prm, count, out))
         return out
