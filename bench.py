@@ -34,8 +34,8 @@ FP64_PEAK_TFLOPS = 78.6
 # FETCH_SIZE and WRITE_SIZE (one pass each) and scaled by what the same
 # counters read for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0); keyed by
 # (kernel, order, triangles, precision). The commands and counter values are in
-# profiles/euler_stage_p2_32k_fp64.md, profiles/euler_stage_mx_p3_524k.md and
-# profiles/advect_stage_524k.md.
+# profiles/euler_stage_p2_32k_fp64.md, profiles/euler_stage_mx_p3_524k.md,
+# profiles/advect_stage_524k.md and profiles/euler_line_stage_1m.md.
 # Other workloads report null.
 MEASURED_TRAFFIC = {
     ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7895.22 * 2.0 + 6149.39) * 1024.0,
@@ -43,6 +43,7 @@ MEASURED_TRAFFIC = {
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (230474.0 * 2.0 + 262229.0) * 1024.0,
     ("whip::advect_stage<1>", 1, 524288, "fp64"): (26979.5 * 2.0 + 12288.1) * 1024.0,
     ("whip::advect_stage<3>", 3, 524288, "fp64"): (51864.7 * 2.0 + 40960.1) * 1024.0,
+    ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (96774.8 * 2.0 + 98346.8) * 1024.0,
 }
 
 
@@ -278,6 +279,98 @@ def advect2d_run(args) -> int:
     return 0
 
 
+def euler1d_bytes_per_element_stage(Np: int) -> float:
+    """The 1D Euler stage reads its input and writes its output (4 Np 8 B
+    each), reads U^n in two stages of three, 1/J (8 B), and the neighbours'
+    coefficients, which are the same array as the input and count once."""
+    return 32 * Np * (2 + 2 / 3) + 8
+
+
+def sod_run(args) -> int:
+    """Sod shock tube (cases/tutorials/euler/sod-shock-tube): P = --order on
+    --cells uniform cells of [0,1], outflow ends, HLLC, characteristic WENO +
+    positivity limiting (threshold 1e-3), CFL 0.2, FP64, a fixed number of
+    steps from t = 0. value = DOF-updates/s of the stepping loop."""
+    import torch
+
+    import wasl_amd
+
+    p, N, cfl = args.order, args.cells, 0.2
+    tags = np.arange(1, N + 2)
+    x = np.arange(N + 1) / N
+    segs = np.stack([tags[:-1], tags[1:]], 1)
+    g = wasl_amd.Engine(wasl_amd.Mesh.lines(tags, x, segs), wasl_amd.EULER, p,
+                        boundary=(wasl_amd.OUTFLOW, wasl_amd.OUTFLOW), limiting=True)
+    sod = (0, 0, 0, 0, 0, 0, 1.0, 0.0, 0.0, 1.0, 1, 0.5, 1.0, 0, 0, 0, 0.125, 0.0, 0.0, 0.1)
+    g.project(wasl_amd.IC_EULER_REGIONS, sod)
+    g.limit()
+    E, Np = g.E, g.Np
+    t, s, _ = g.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                        stop_at_max_steps=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = E * Np * 4 * args.steps / wall
+    g.set_timing(True)
+    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
+                        stop_at_max_steps=True)
+    g.set_timing(False)
+    _, stage_total, launches = g.last_timing()
+    stage_ms = stage_total / max(launches, 1)
+    bytes_stage = E * euler1d_bytes_per_element_stage(Np)
+    achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
+    log(f"sod P{p}: {E} cells, {wall * 1e3 / args.steps:.4f} ms/step, {value:.4e} DOF-updates/s, "
+        f"stage {stage_ms * 1e3:.2f} us, {achieved:.0f} GB/s, t = {t:.5f}")
+    result = {
+        "metric": "1D Euler (limited Sod) DOF-updates/s",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"1D Euler Sod shock tube, P{p} modal Legendre, {E} uniform cells on "
+                        f"[0,1], outflow ends, HLLC, char-WENO + positivity, CFL {cfl}, FP64",
+            "elements": E, "modes": Np, "variables": 4, "dofs": E * Np * 4,
+            "state_finite": bool(np.isfinite(g.state).all()),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get((f"whip::euler_line_stage<{p}>", p, E, "fp64")),
+            "kernel": f"whip::euler_line_stage<{p}>", "stage_kernel_us": stage_ms * 1e3,
+            "algorithmic_bytes_per_launch": bytes_stage,
+        },
+        "cpu_baseline": None,
+    }
+    if result["roofline"]["traffic"] is not None:
+        result["roofline"]["traffic_frac_of_algorithmic"] = (
+            result["roofline"]["traffic"] / bytes_stage)
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        kind = "reference" if O.available("reference") else "port"
+        c = O.line_euler(tags, x, segs, p, left=O.OUTFLOW, right=O.OUTFLOW, kind=kind)
+        U, _ = c.limit(c.project(O.IC_EULER_REGIONS, sod), 1e-3)
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            U = c.step(U, c.stable_dt(U, cfl), True, 1e-3)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > args.cpu_seconds or steps >= 200:
+                break
+        result["cpu_baseline"] = {
+            "value": c.E * c.Np * 4 * steps / el, "unit": "DOF-updates/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{steps} limited SSP-RK3 steps of the same P{p} {c.E}-cell Sod case "
+                      f"({el:.1f} s, single thread, {cpu_model()})",
+        }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
@@ -384,11 +477,12 @@ def main(argv=None) -> int:
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests of the serving mix (0: skip the serving measurement)")
     ap.add_argument("--serving-seed", type=int, default=0x5EED)
-    ap.add_argument("--workload", choices=["vortex", "rarefaction", "advect2d"], default="vortex",
+    ap.add_argument("--workload", choices=["vortex", "rarefaction", "advect2d", "sod"],
+                    default="vortex",
                     help="vortex: the headline (config 2 sizes by default); rarefaction: the "
                          "limited radial rarefaction run to --end-time; advect2d: 2D scalar "
-                         "advection at --order and --cells; each other workload prints one "
-                         "JSON line of its own")
+                         "advection at --order and --cells; sod: 1D Euler Sod tube, --cells "
+                         "cells; each other workload prints one JSON line of its own")
     ap.add_argument("--end-time", type=float, default=0.3)
     ap.add_argument("--error-steps", type=int, default=20,
                     help="steps of the reduced-precision error check (--precision mixed/bf16)")
@@ -426,6 +520,8 @@ def main(argv=None) -> int:
 
     if args.workload == "advect2d":
         return advect2d_run(args)
+    if args.workload == "sod":
+        return sod_run(args)
 
     if args.workload == "rarefaction":
         res = rarefaction_run(args.cells, args.end_time)
