@@ -125,8 +125,9 @@ int wasl_hip_mesh_project(const wasl_hip_mesh *mesh, int polynomial_order, doubl
  * (include/wasl/case/euler_case.hpp:28-42, scalar_advection_case.hpp:29-45)
  * that reach the hot path. */
 typedef struct {
-  int32_t equation;          /* WASL_HIP_EULER (2D triangles) or WASL_HIP_SCALAR_ADVECTION (1D lines,
-                                or 2D periodic triangles: scalar_advection_triangle.cpp:15-46) */
+  int32_t equation;          /* WASL_HIP_EULER: triangles (2D) or lines (1D, euler_line.cpp:14-48,
+                                periodic or outflow ends, FP64); WASL_HIP_SCALAR_ADVECTION: lines
+                                (1D) or periodic triangles (2D, scalar_advection_triangle.cpp:15-46) */
   int32_t polynomial_order;  /* 0..3 */
   double heat_capacity_ratio;
   double wavespeed;          /* scalar advection a (1D), velocity_x (2D) */
@@ -167,7 +168,8 @@ int64_t wasl_hip_engine_size(const wasl_hip_engine *engine, int what);
 int wasl_hip_engine_trace(const wasl_hip_engine *engine, int32_t *donors, double *basis);
 
 /* Limiter stencil (EulerLimiterStencil, include/wasl/limiting/euler_limiter_stencil.hpp:19-50):
- * neighbors [E][3], transforms [E][3][Np][Np], normals [E][2],
+ * neighbors [E][K], transforms [E][K][Np][Np], normals [E][2] with K = 3 on
+ * triangles and 2 on lines (euler_limiter_stencil_line.cpp:79-94),
  * check_basis [C][Np] with C = wasl_hip_engine_size(engine, 7). */
 int wasl_hip_engine_stencil(const wasl_hip_engine *engine, int32_t *neighbors, double *transforms,
                             double *normals, double *check_basis);

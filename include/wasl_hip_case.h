@@ -46,7 +46,8 @@ void wasl_hip_string_free(char *text);
  * 1 inflow, 2 outflow. Summary "step=<n> time=<%.17g> checkpoint=<path>"
  * (bridge.cpp:40-46). 2D runs need periodic boundaries as in the reference
  * (src/case/euler_triangle.cpp:17-18), or, beyond the reference, every boundary
- * outflow. dimensions = 1 (Euler on lines) is not part of this engine and fails. */
+ * outflow. dimensions = 1 runs Euler on lines (src/case/euler_line.cpp:14-48):
+ * periodic or outflow ends; inflow fails with the reference's message. */
 int wasl_hip_run_euler_case(const char *mesh_path, const char *output_path,
                             const char *restart_path, int limiting, double sensor_threshold,
                             double density_floor, double pressure_floor, int dimensions,
