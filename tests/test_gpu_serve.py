@@ -137,3 +137,21 @@ def test_failure_stays_local(gpu):
     ref = _run(gpu, reqs)
     for a, b in zip(out[:-1], ref):
         assert np.array_equal(a[0], b[0])
+
+
+def test_unserved_kinds_are_refused(gpu):
+    """Serving takes the workload mix's two kinds (2D Euler, 1D advection);
+    1D Euler and 2D advection cases are refused at submission, with a message,
+    and run on their own engines."""
+    from wasl_amd.serve import Server
+
+    s = Server()
+    lines = gpu.Mesh.from_input(MG.jittered_line(50))
+    tris = gpu.Mesh.from_input(K.vortex_mesh(4))
+    with pytest.raises(gpu.WaslHipError, match="serving takes"):
+        s.submit(lines, gpu.EULER, 1, gpu.IC_EULER_REGIONS, K.SHOCK_TUBE, end_steps=5)
+    with pytest.raises(gpu.WaslHipError, match="serving takes"):
+        s.submit(tris, gpu.SCALAR_ADVECTION, 1, gpu.IC_SCALAR_WAVE, K.SCALAR_WAVE, end_steps=5)
+    ok = s.submit(lines, gpu.SCALAR_ADVECTION, 1, gpu.IC_SCALAR_PULSE, (), end_steps=5, cfl=0.1)
+    assert s.drain()[ok][2] == 5
+    s.close()

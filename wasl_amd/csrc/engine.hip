@@ -606,7 +606,7 @@ template <int P>
 void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip::RunCtl *ctl,
                     int parity) {
   const whip::LimiterDev L = g.limiter();
-  const dim3 grid(whip::limit_blocks(g.E)), grid256((g.E + 255) / 256);
+  const dim3 grid(whip::limit_blocks(g.E)), grid256(whip::kScatterBlocks);
   if (g.dim == 1) {
     hipLaunchKernelGGL((whip::limit_troubled_line<P>), grid, dim3(64), 0, g.stream, L, U,
                        g.side.as<double>(), g.list.as<int>(), count, ctl, parity,

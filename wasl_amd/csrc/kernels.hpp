@@ -2605,13 +2605,41 @@ __global__ __launch_bounds__(64) void limit_troubled(LimiterDev L, const double 
   limit_wave_body<P>(L, U, side, list, count, ctl, parity, err, blockIdx.x, gridDim.x);
 }
 
+// The single-case scatter runs on a small fixed grid that strides over the
+// troubled list: the list is short next to E, and one thread per element
+// would cost more in launch than the copy itself.
+constexpr int kScatterBlocks = 32;
+
+// Flattened over (slot, variable, mode): side[slot][4][Np] goes to
+// U[variable][mode][list[slot]].
+template <int Np>
+__device__ __forceinline__ void scatter_strided(double *U, int E, const double *side,
+                                                const int *list, const int *count, RunCtl *ctl,
+                                                int parity) {
+  if (ctl && (ctl->err || !ctl->s[parity].active))
+    return;
+  const int n = *count * 4 * Np;
+  const int stride = static_cast<int>(gridDim.x * blockDim.x);
+  for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+    const int slot = i / (4 * Np), r = i - slot * 4 * Np;
+    U[static_cast<size_t>(r) * static_cast<unsigned>(E) + list[slot]] = side[i];
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
                                                        const int *list, const int *count,
                                                        RunCtl *ctl, int parity, float *tr,
                                                        const float *mxtab) {
-  scatter_body<P>(U, E, side, list, count, ctl, parity, blockIdx.x * blockDim.x + threadIdx.x,
-                  tr, mxtab);
+  if (!tr) {
+    scatter_strided<Tri<P>::Np>(U, E, side, list, count, ctl, parity);
+    return;
+  }
+  // mixed precision: one element per thread (its FP32 traces are refreshed too)
+  const int stride = static_cast<int>(gridDim.x * blockDim.x);
+  for (int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); idx < *count;
+       idx += stride)
+    scatter_body<P>(U, E, side, list, count, ctl, parity, idx, tr, mxtab);
 }
 
 // Lines: two neighbours, (P+1) Gauss points and the two ends as check points.
@@ -2628,8 +2656,7 @@ template <int P>
 __global__ __launch_bounds__(256) void scatter_limited_line(double *U, int E, const double *side,
                                                             const int *list, const int *count,
                                                             RunCtl *ctl, int parity) {
-  scatter_core<P + 1>(U, E, side, list, count, ctl, parity,
-                      blockIdx.x * blockDim.x + threadIdx.x);
+  scatter_strided<P + 1>(U, E, side, list, count, ctl, parity);
 }
 
 // Batched limiter: tiles of 64 troubled-list slots per (case, chunk); the
