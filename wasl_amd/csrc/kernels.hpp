@@ -547,6 +547,9 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
             (v * LANES / 4 == sub && a.ca != 0.0) ? at_row(a.un, E, v * Np + m, e) : 0.0;
   }
 
+  // 1/detJ is requested here (from P2 on), its latency overlapping the faces.
+  const double invdet = kEarlyMetrics ? gm[4] : at_row(mesh.geo, E, 4, e);
+
   // Faces: each lane takes whole edges; traces on both sides use the left
   // side's quadrature point and HLLC is always evaluated (left, right,
   // n_left), so the two elements of a face see identical flux bits (exact
@@ -643,37 +646,49 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   const int el_w = static_cast<int>(threadIdx.x) / LANES - wv * EPW;   // element within the wave
 
   // Finalise: lane `sub` owns variables [sub*4/LANES, (sub+1)*4/LANES).
-  const double invdet = kEarlyMetrics ? gm[4] : at_row(mesh.geo, E, 4, e);
   double out0[4] = {0.0, 0.0, 0.0, 0.0};
   double dens[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m)
     dens[m] = 0.0;
+  // One loop per source of U^n. A single loop choosing between the LDS slab
+  // and global memory compiles to flat loads, and every flat load waits for
+  // all outstanding vector-memory operations, the previous store included.
+  const auto finalize = [&](auto un_at) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    if (v * LANES / 4 != sub)
-      continue;
+    for (int v = 0; v < 4; ++v) {
+      if (v * LANES / 4 != sub)
+        continue;
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      const double r = acc[v][m] * invdet;
-      const int idx = (v * Np + m) * E + e;
-      double y;
-      if (a.residual_only) {
-        y = r;
-      } else {
-        y = a.cb * (c[v][m] + dt * r);
-        if (a.ca != 0.0)
-          y += a.ca * (kEarlyMetrics ? unr[v][m]
-                       : un_lds   ? unl[kUnLds ? wv : 0][kUnLds ? (v * Np + m) * EPW + el_w : 0]
-                                  : a.un[idx]);
+      for (int m = 0; m < Np; ++m) {
+        const double r = acc[v][m] * invdet;
+        const int idx = (v * Np + m) * E + e;
+        double y;
+        if (a.residual_only) {
+          y = r;
+        } else {
+          y = a.cb * (c[v][m] + dt * r);
+          if (a.ca != 0.0)
+            y += a.ca * un_at(v, m, idx);
+        }
+        if (valid)
+          a.out[idx] = y;
+        if (m == 0)
+          out0[v] = y;
+        if (v == 0)
+          dens[m] = y;
       }
-      if (valid)
-        a.out[idx] = y;
-      if (m == 0)
-        out0[v] = y;
-      if (v == 0)
-        dens[m] = y;
     }
+  };
+  if constexpr (kEarlyMetrics) {
+    finalize([&](int v, int m, int) { return unr[v][m]; });
+  } else if constexpr (kUnLds) {
+    if (un_lds)
+      finalize([&](int v, int m, int) { return unl[wv][(v * Np + m) * EPW + el_w]; });
+    else
+      finalize([&](int, int, int idx) { return a.un[idx]; });
+  } else {
+    finalize([&](int, int, int idx) { return a.un[idx]; });
   }
 
   // Gather the new cell averages to every lane of the element: only the
