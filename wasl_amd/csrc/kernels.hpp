@@ -374,9 +374,15 @@ __device__ __forceinline__ void lds_copy16(const void *src, void *dst) {
 // Element at a 32-bit byte offset from a wave-uniform base: the load takes
 // the scalar-base form (base in scalar registers, one 32-bit offset register
 // per lane shared by every row), with no per-load address arithmetic.
+//
+// The access is made through a global-address-space pointer: every array
+// read this way is device memory, and a pointer the kernel read from memory
+// (a batch's case table) would otherwise compile to flat loads, which wait
+// for LDS and vector memory alike.
 template <typename T>
 __device__ __forceinline__ T at_off(const T *base, unsigned byte_off) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+  using G = const __attribute__((address_space(1))) T;
+  return *(G *)(reinterpret_cast<const char *>(base) + byte_off);
 }
 
 // Row r of an SoA array of E columns at column i (r wave-uniform).
@@ -948,20 +954,26 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
 #pragma unroll
       for (int m = 0; m < Np; ++m)
         si += c[m] * own[q * Np + m];
+      // The interior trace always runs and a boundary face overwrites it: a
+      // branch between the LDS tables and the global donor basis would
+      // compile to flat loads.
       double so = 0.0;
-      if (!boundary) {
 #pragma unroll
-        for (int m = 0; m < Np; ++m)
-          so += nv[m] * oth[q * Np + m];
-      } else if (mesh.bdonor) {
-        const int slot = meta_slot(meta);
-        const int d = mesh.bdonor[slot * NE + q];
-        const double *db = mesh.bbasis + (slot * NE + q) * Np;
+      for (int m = 0; m < Np; ++m)
+        so += nv[m] * oth[q * Np + m];
+      if (boundary) {
+        if (mesh.bdonor) {
+          const int slot = meta_slot(meta);
+          const int d = mesh.bdonor[slot * NE + q];
+          const double *db = mesh.bbasis + (slot * NE + q) * Np;
+          double sd = 0.0;
 #pragma unroll
-        for (int m = 0; m < Np; ++m)
-          so += a.uin[(v * Np + m) * E + d] * db[m];
-      } else {
-        so = si;  // transmissive: exterior = own trace
+          for (int m = 0; m < Np; ++m)
+            sd += a.uin[(v * Np + m) * E + d] * db[m];
+          so = sd;
+        } else {
+          so = si;  // transmissive: exterior = own trace
+        }
       }
       xq[q][v] = si;
       xo[q][v] = so;
@@ -1662,17 +1674,41 @@ struct BatchStage {
   int stage, parity;
 };
 
+// A pointer read from memory (a batch's case table) is generic to the
+// compiler: every access through it becomes a flat instruction, and a flat
+// load waits for LDS and vector memory alike (stores included). These casts
+// state that the case buffers are global memory.
+template <typename T> __device__ __forceinline__ T *global_ptr(T *p) {
+  using G = __attribute__((address_space(1))) T;
+  return (T *)((G *)reinterpret_cast<uintptr_t>(p));
+}
+
+__device__ __forceinline__ EulerMesh global_mesh(EulerMesh m) {
+  m.geo = global_ptr(m.geo);
+  m.nbr = global_ptr(m.nbr);
+  m.meta = global_ptr(m.meta);
+  m.fn = global_ptr(m.fn);
+  m.bdonor = global_ptr(m.bdonor);
+  m.bbasis = global_ptr(m.bbasis);
+  m.table = global_ptr(m.table);
+  m.vtable = global_ptr(m.vtable);
+  return m;
+}
+
 // Stage buffers and SSP-RK3 (Shu-Osher) weights of one stage: stage s reads
 // {u, w1, w2}[s] and writes {w1, w2, u}[s], out = ca U^n + cb (in + dt R).
 __device__ __forceinline__ StageArgs stage_args(double *u, double *w1, double *w2, RunCtl *ctl,
                                                 BatchStage b) {
   StageArgs a{};
+  u = global_ptr(u);
+  w1 = global_ptr(w1);
+  w2 = global_ptr(w2);
   a.uin = b.stage == 0 ? u : b.stage == 1 ? w1 : w2;
   a.out = b.stage == 0 ? w1 : b.stage == 1 ? w2 : u;
   a.un = u;
   a.ca = b.stage == 0 ? 0.0 : b.stage == 1 ? 0.75 : 1.0 / 3.0;
   a.cb = b.stage == 0 ? 1.0 : b.stage == 1 ? 0.25 : 2.0 / 3.0;
-  a.ctl = ctl;
+  a.ctl = global_ptr(ctl);
   a.parity = b.parity;
   a.stage = b.stage;
   return a;
@@ -1684,10 +1720,10 @@ __device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b
   a.sensor = c.sensor;
   a.threshold = c.threshold;
   a.highest_begin = c.highest_begin;
-  a.list = c.list;
-  a.counters = c.counters;
-  a.tin = b.stage == 0 ? c.tu : b.stage == 1 ? c.tw1 : c.tw2;
-  a.tout = b.stage == 0 ? c.tw1 : b.stage == 1 ? c.tw2 : c.tu;
+  a.list = global_ptr(c.list);
+  a.counters = global_ptr(c.counters);
+  a.tin = global_ptr(b.stage == 0 ? c.tu : b.stage == 1 ? c.tw1 : c.tw2);
+  a.tout = global_ptr(b.stage == 0 ? c.tw1 : b.stage == 1 ? c.tw2 : c.tu);
   return a;
 }
 
@@ -1701,7 +1737,7 @@ void euler_stage_mx_batch(const BatchCase *cases, const int2 *tiles, BatchStage 
   if (t.x < 0)
     return;   // padding tile
   const BatchCase &c = cases[t.x];
-  mx_stage<P, BF16>(c.mesh, batch_args(c, b), mxtab, t.y, t.y == 0, tab, red);
+  mx_stage<P, BF16>(global_mesh(c.mesh), batch_args(c, b), mxtab, t.y, t.y == 0, tab, red);
 }
 
 // FP64 point split over a batch: tiles index workgroups of 256 / LANES
@@ -1713,7 +1749,7 @@ void euler_stage_batch(const BatchCase *cases, const int2 *tiles, BatchStage b) 
   if (t.x < 0)
     return;   // padding tile
   const BatchCase &c = cases[t.x];
-  euler_stage_body<P, LANES>(c.mesh, batch_args(c, b), t.y, t.y == 0);
+  euler_stage_body<P, LANES>(global_mesh(c.mesh), batch_args(c, b), t.y, t.y == 0);
 }
 
 // FP64 variable split over a batch (H quads per element, 64 / H elements per
@@ -1726,7 +1762,7 @@ void euler_stage_vs_batch(const BatchCase *cases, const int2 *tiles, BatchStage 
   if (t.x < 0)
     return;   // padding tile
   const BatchCase &c = cases[t.x];
-  euler_stage_vs_body<P, H>(c.mesh, batch_args(c, b), t.y, t.y == 0, sh);
+  euler_stage_vs_body<P, H>(global_mesh(c.mesh), batch_args(c, b), t.y, t.y == 0, sh);
 }
 
 // ------------------------------------------------- 1D scalar advection ----
@@ -1853,7 +1889,10 @@ __global__ __launch_bounds__(256) void line_stage_batch(const BatchLine *cases, 
   if (t.x < 0)
     return;   // padding tile
   const BatchLine &c = cases[t.x];
-  line_body<P>(c.mesh, stage_args(c.u, c.w1, c.w2, c.ctl, b), t.y, t.y == 0, tab);
+  LineMeshDev m = c.mesh;
+  m.inv_jac = global_ptr(m.inv_jac);
+  m.table = global_ptr(m.table);
+  line_body<P>(m, stage_args(c.u, c.w1, c.w2, c.ctl, b), t.y, t.y == 0, tab);
 }
 
 // ------------------------------------------- 2D scalar advection stage ----
@@ -2687,11 +2726,19 @@ struct BatchLimit {
 };
 
 __device__ __forceinline__ double *batch_out(const BatchLimit &c, int stage) {
-  return stage == 0 ? c.w1 : stage == 1 ? c.w2 : c.u;
+  return global_ptr(stage == 0 ? c.w1 : stage == 1 ? c.w2 : c.u);
 }
 
 __device__ __forceinline__ float *batch_out_traces(const BatchLimit &c, int stage) {
-  return stage == 0 ? c.tw1 : stage == 1 ? c.tw2 : c.tu;
+  return global_ptr(stage == 0 ? c.tw1 : stage == 1 ? c.tw2 : c.tu);
+}
+
+__device__ __forceinline__ LimiterDev global_limiter(LimiterDev L) {
+  L.nbr = global_ptr(L.nbr);
+  L.T = global_ptr(L.T);
+  L.nrm = global_ptr(L.nrm);
+  L.check = global_ptr(L.check);
+  return L;
 }
 
 // Wave-per-element limiter blocks of one case in a batch (tiles (case, b),
@@ -2705,8 +2752,9 @@ __global__ __launch_bounds__(64) void limit_troubled_batch(const BatchLimit *cas
   if (t.x < 0)
     return;   // padding tile
   const BatchLimit &c = cases[t.x];
-  limit_wave_body<P>(c.L, batch_out(c, b.stage), c.side, c.list, c.counters + b.stage, c.ctl,
-                     b.parity, c.err, t.y, limit_blocks(c.L.E));
+  limit_wave_body<P>(global_limiter(c.L), batch_out(c, b.stage), global_ptr(c.side),
+                     global_ptr(c.list), global_ptr(c.counters) + b.stage, global_ptr(c.ctl),
+                     b.parity, global_ptr(c.err), t.y, limit_blocks(c.L.E));
 }
 
 template <int P>
@@ -2717,8 +2765,9 @@ __global__ __launch_bounds__(64) void scatter_limited_batch(const BatchLimit *ca
   if (t.x < 0)
     return;   // padding tile
   const BatchLimit &c = cases[t.x];
-  scatter_body<P>(batch_out(c, b.stage), c.L.E, c.side, c.list, c.counters + b.stage, c.ctl,
-                  b.parity, t.y * 64 + threadIdx.x, batch_out_traces(c, b.stage), mxtab);
+  scatter_body<P>(batch_out(c, b.stage), c.L.E, global_ptr(c.side), global_ptr(c.list),
+                  global_ptr(c.counters) + b.stage, global_ptr(c.ctl), b.parity,
+                  t.y * 64 + threadIdx.x, batch_out_traces(c, b.stage), mxtab);
 }
 
 }  // namespace whip
