@@ -1468,14 +1468,29 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const int d = mesh.bdonor[slot * NE + s];
       const double *db = mesh.bbasis + (slot * NE + s) * Np;
       const unsigned doff = static_cast<unsigned>(d) * 8u;
+      // The donor basis and a variable's modes (half of them from P3) are
+      // requested together, so a wave waits once per group, not per load.
+      float bm[Np];
+#pragma unroll
+      for (int m = 0; m < Np; ++m)
+        bm[m] = static_cast<float>(db[m]);
       float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      constexpr int kChunk = Np > 6 ? (Np + 1) / 2 : Np;
 #pragma unroll
-      for (int m = 0; m < Np; ++m) {
-        const float bm = static_cast<float>(db[m]);
+      for (int v = 0; v < 4; ++v) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          x[v] += static_cast<float>(ld_f64(rin, doff, static_cast<unsigned>(v * Np + m) * Eu * 8u)) *
-                  bm;
+        for (int m0 = 0; m0 < Np; m0 += kChunk) {
+          double cv[kChunk];
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i)
+            cv[i] = m0 + i < Np
+                        ? ld_f64(rin, doff, static_cast<unsigned>(v * Np + m0 + i) * Eu * 8u)
+                        : 0.0;
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i)
+            if (m0 + i < Np)
+              x[v] += static_cast<float>(cv[i]) * bm[m0 + i];
+        }
       }
       oth = {x[0], x[1], x[2], x[3]};
     }
