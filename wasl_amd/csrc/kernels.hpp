@@ -1141,9 +1141,16 @@ constexpr int kMetaReverse = 128;   // the two sides order the face points oppos
 template <int P> struct Mx {
   static constexpr int Np = Tri<P>::Np, NQ = Tri<P>::NQ, NE = Tri<P>::NE;
   static constexpr int MT = (NQ + 15) / 16;   // point tiles of 16
-  // K steps over modes: step r takes modes 4g + r from row group g, so a lane
-  // feeds its own modes 4g .. 4g + 3, the layout the epilogue holds
-  static constexpr int KS = Np < 4 ? Np : 4;
+  // K steps over modes: row group g owns modes KS g .. KS g + KS - 1 and step
+  // r takes mode KS g + r from it, so a lane feeds its own modes. Rows of the
+  // residual products follow the same map (row 4g + r is mode KS g + r, rows
+  // with r >= KS are padding), so the lane that holds a mode's operand also
+  // receives its residual, the layout the epilogue needs.
+  static constexpr int KS = (Np + 3) / 4;
+  // mode of residual row `row`, Np for a padding row
+  static constexpr int row_mode(int row) {
+    return (row & 3) < KS && KS * (row >> 2) + (row & 3) < Np ? KS * (row >> 2) + (row & 3) : Np;
+  }
   // per-lane operand tables, floats, each [..][64]
   static constexpr int kAvol = 0;                         // [MT][KS]
   static constexpr int kAgrad = kAvol + MT * KS * 64;     // [MT][4 r][2 d]
@@ -1162,7 +1169,11 @@ typedef float mx_f4 __attribute__((ext_vector_type(4)));
 
 // Waves per SIMD the mixed stage kernels are built for (register budget 512 /
 // waves): every global read sits in registers from the start of the kernel.
-constexpr int kMxWaves = 3;
+// Measured on MI355X: five waves at P1 and four at P2 beat three (7-14 %);
+// P3 needs the registers of three.
+template <int P, bool BF16> constexpr int mx_waves() {
+  return P <= 1 ? 5 : P == 2 ? 4 : 3;
+}
 typedef short mx_s4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ mx_f4 mfma4(float a, float b, mx_f4 c) {
@@ -1272,14 +1283,14 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   const size_t state_bytes = static_cast<size_t>(4 * Np) * Eu * 8;
   const Rsrc rin = rsrc(a.uin, state_bytes);
   const unsigned eoff = static_cast<unsigned>(e) * 8u;
-  const unsigned goff = static_cast<unsigned>(4 * grp) * Eu * 8u + eoff;
-  double c64[4][4], un64[4][4];
-  float cf[4][4];
+  const unsigned goff = static_cast<unsigned>(KS * grp) * Eu * 8u + eoff;
+  double c64[4][KS], un64[4][KS];
+  float cf[4][KS];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const bool real = 4 * grp + r < Np;
+    for (int r = 0; r < KS; ++r) {
+      const bool real = KS * grp + r < Np;
       const double x = ld_f64(rin, real ? goff : eoff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
       c64[v][r] = real ? x : 0.0;
       cf[v][r] = static_cast<float>(c64[v][r]);
@@ -1353,8 +1364,9 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     if constexpr (BF16) {
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        cb16[v] = mx_s4{to_bf16(cf[v][0]), to_bf16(cf[v][1]), to_bf16(cf[v][2]),
-                        to_bf16(cf[v][3])};
+#pragma unroll
+        for (int c = 0; c < 4; ++c)   // K entries 4 grp + c; c >= KS is padding
+          cb16[v][c] = c < KS ? to_bf16(cf[v][c < KS ? c : 0]) : short(0);
     }
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
@@ -1421,14 +1433,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   }
 
   // U^n of the owned outputs (stages 2 and 3): requested here, while the
-  // faces are evaluated, rather than held through the volume phase.
+  // faces are evaluated, rather than held through the volume phase, and used
+  // only in the epilogue, so that its latency overlaps the face work.
   if (a.ca != 0.0) {
     const Rsrc run = rsrc(a.un, state_bytes);
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool real = 4 * grp + r < Np;
+      for (int r = 0; r < KS; ++r) {
+        const bool real = KS * grp + r < Np;
         const double x =
             ld_f64(run, real ? goff : eoff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
         un64[v][r] = real ? x : 0.0;
@@ -1437,7 +1450,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < KS; ++r)
         un64[v][r] = 0.0;
   }
 
@@ -1509,16 +1522,17 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       R[v] = mfma4(al, fw[v], R[v]);
   }
 
-  // ---- finalise: lane holds R[v][m = 4 grp + r] of element j, FP64 from here
-  float yf[4][4];
+  // ---- finalise: lane holds R[v][row 4 grp + r] of element j, mode KS grp + r
+  // for r < KS; FP64 from here
+  float yf[4][KS];
   double avg[4] = {0.0, 0.0, 0.0, 0.0};
   double total = 0.0, high = 0.0;
   const Rsrc rout = rsrc(a.out, state_bytes);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = 4 * grp + r;
+    for (int r = 0; r < KS; ++r) {
+      const int m = KS * grp + r;
       yf[v][r] = 0.0f;
       if (m < Np) {
         const double res = static_cast<double>(R[v][r]) * invdet;
@@ -1637,7 +1651,7 @@ __device__ __forceinline__ void mx_traces_body(const double *U, int E, float *T,
     const float af = mxtab[X::kAface + s * 64 + lane];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int m = 4 * grp + s;
+      const int m = KS * grp + s;
       const float c = m < Np ? static_cast<float>(U[(v * Np + m) * E + e]) : 0.0f;
       Tn[v] = mfma4(af, c, Tn[v]);
     }
@@ -1657,7 +1671,7 @@ __global__ __launch_bounds__(256) void mx_traces(const double *U, int E, float *
 }
 
 template <int P, bool BF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMxWaves)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mx_waves<P, BF16>())))
 void euler_stage_mx(EulerMesh mesh, StageArgs a, const float *mxtab) {
   __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
   __shared__ double red[4];
@@ -1743,7 +1757,7 @@ __device__ __forceinline__ StageArgs batch_args(const BatchCase &c, BatchStage b
 }
 
 template <int P, bool BF16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMxWaves)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mx_waves<P, BF16>())))
 void euler_stage_mx_batch(const BatchCase *cases, const int2 *tiles, BatchStage b,
                           const float *mxtab) {
   __shared__ __align__(16) float tab[mx_staged<P, BF16>()];
