@@ -371,7 +371,8 @@ def sod_run(args) -> int:
     return 0
 
 
-def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dict:
+def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
+                        setup_threads: int = 4) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
     and time to solution under Poisson arrivals at 80 % of the served
@@ -399,7 +400,8 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dic
     log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s"
         f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
 
-    done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch)
+    done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch,
+                                          setup_threads=setup_threads)
     served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
     served_rate = served_updates / makespan
     capacity = count / makespan
@@ -410,7 +412,8 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16) -> dic
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
     done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals,
-                                          steps_per_batch=steps_per_batch)
+                                          steps_per_batch=steps_per_batch,
+                                          setup_threads=setup_threads)
     tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
     plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
 
@@ -488,6 +491,8 @@ def main(argv=None) -> int:
                     help="steps of the reduced-precision error check (--precision mixed/bf16)")
     ap.add_argument("--serving-steps", type=int, default=16,
                     help="SSP-RK3 steps per scheduling round of the server (even)")
+    ap.add_argument("--serving-threads", type=int, default=4,
+                    help="host threads that prepare served cases")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -630,7 +635,7 @@ def main(argv=None) -> int:
     if args.serving_requests > 0:
         del eng
         result["serving"] = serving_measurement(args.serving_requests, args.serving_seed,
-                                                args.serving_steps)
+                                                args.serving_steps, args.serving_threads)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
         result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)
