@@ -418,18 +418,20 @@ template <int P> std::vector<float> mixed_tables() {
     const int i = l & 15, kk = l >> 4;
     for (int tt = 0; tt < MT; ++tt) {
       for (int s = 0; s < KS; ++s)
-        t[X::kAvol + (tt * KS + s) * 64 + l] = static_cast<float>(vb(16 * tt + i, KS * kk + s));
+        t[X::kAvol + (tt * KS + s) * 64 + l] =
+            static_cast<float>(vb(X::point(tt, i), KS * kk + s));
+      // K step r of tile tt takes point X::point(tt, 4 kk + r) from row group kk
       for (int r = 0; r < 4; ++r)
         for (int d = 0; d < 2; ++d)
           t[X::kAgrad + ((tt * 4 + r) * 2 + d) * 64 + l] =
-              static_cast<float>(wd(d, 16 * tt + 4 * kk + r, X::row_mode(i)));
+              static_cast<float>(wd(d, X::point(tt, 4 * kk + r), X::row_mode(i)));
       // bf16: A[i][4 kk + c] with four consecutive K entries per lane; in the
       // volume product K entry 4 kk + c is mode KS kk + c (c < KS)
       uint16_t hv[4], hx[4], he[4];
       for (int c = 0; c < 4; ++c) {
-        hv[c] = bf16_bits(static_cast<float>(c < KS ? vb(16 * tt + i, KS * kk + c) : 0.0));
-        hx[c] = bf16_bits(static_cast<float>(wd(0, 16 * tt + 4 * kk + c, X::row_mode(i))));
-        he[c] = bf16_bits(static_cast<float>(wd(1, 16 * tt + 4 * kk + c, X::row_mode(i))));
+        hv[c] = bf16_bits(static_cast<float>(c < KS ? vb(X::point(tt, i), KS * kk + c) : 0.0));
+        hx[c] = bf16_bits(static_cast<float>(wd(0, X::point(tt, 4 * kk + c), X::row_mode(i))));
+        he[c] = bf16_bits(static_cast<float>(wd(1, X::point(tt, 4 * kk + c), X::row_mode(i))));
       }
       std::memcpy(&t[X::kBvol16 + (tt * 64 + l) * 2], hv, sizeof hv);
       std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 0) * 64 + l) * 2], hx, sizeof hx);

@@ -1141,6 +1141,11 @@ constexpr int kMetaReverse = 128;   // the two sides order the face points oppos
 template <int P> struct Mx {
   static constexpr int Np = Tri<P>::Np, NQ = Tri<P>::NQ, NE = Tri<P>::NE;
   static constexpr int MT = (NQ + 15) / 16;   // point tiles of 16
+  // Row 4g + r of point tile t is point 16t + 4r + g: the real points of the
+  // last tile fill whole K steps r of the flux product, and the steps with no
+  // real point (16t + 4r >= NQ) are left out.
+  static constexpr int point(int t, int row) { return 16 * t + 4 * (row & 3) + (row >> 2); }
+  static constexpr bool step_real(int t, int r) { return 16 * t + 4 * r < NQ; }
   // K steps over modes: row group g owns modes KS g .. KS g + KS - 1 and step
   // r takes mode KS g + r from it, so a lane feeds its own modes. Rows of the
   // residual products follow the same map (row 4g + r is mode KS g + r, rows
@@ -1391,7 +1396,9 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       mx_f4 Fxi[4], Feta[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const bool real = 16 * t + 4 * grp + r < NQ;   // padded points: zero fluxes
+        if (!BF16 && !X::step_real(t, r))
+          continue;   // no real point in this K step: the step is not issued
+        const bool real = X::point(t, 4 * grp + r) < NQ;   // padded points: zero fluxes
         const float rho = S[0][r], mx = S[1][r], my = S[2][r], en = S[3][r];
         const float inv = rcpf(real ? rho : 1.0f);
         const float u = mx * inv, w = my * inv;
@@ -1420,6 +1427,8 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+          if (!X::step_real(t, r))
+            continue;
           const float ax = tab[X::kAgrad + ((t * 4 + r) * 2 + 0) * 64 + lane];
           const float ae = tab[X::kAgrad + ((t * 4 + r) * 2 + 1) * 64 + lane];
 #pragma unroll
