@@ -372,7 +372,7 @@ def sod_run(args) -> int:
 
 
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
-                        setup_threads: int = 4) -> dict:
+                        setup_threads: int = 4, fp64_tiles=None) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
     and time to solution under Poisson arrivals at 80 % of the served
@@ -380,6 +380,7 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
     import numpy as np
 
     from wasl_amd import workload as WL
+    from wasl_amd.serve import default_fp64_tiles
 
     mix = WL.serving_mix(count, seed)
     meshes = WL.MeshCache()
@@ -401,7 +402,7 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
 
     done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads)
+                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles)
     served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
     served_rate = served_updates / makespan
     capacity = count / makespan
@@ -413,7 +414,7 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
     done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals,
                                           steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads)
+                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles)
     tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
     plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
 
@@ -443,6 +444,10 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         "served_precision": "unlimited 2D Euler P>=2: mixed (FP32 matrix-core contractions, "
                             "FP64 state and RK); P<=1 and limited cases: FP64 stage; limiter, "
                             "CFL and 1D advection FP64",
+        # unlimited FP64 classes; limited cases keep each case's autotuned tile
+        "served_fp64_tiles": {f"P{o}": ({"stage_kernel": k, "lanes": n} if k else "autotuned")
+                              for o, (k, n) in sorted({**default_fp64_tiles(),
+                                                       **(fp64_tiles or {})}.items())},
         "capacity_requests_per_s": capacity,
         "arrival_rate_per_s": rate,
         "tts_p50_s": WL.percentile(tts, 50),
@@ -493,6 +498,10 @@ def main(argv=None) -> int:
                     help="SSP-RK3 steps per scheduling round of the server (even)")
     ap.add_argument("--serving-threads", type=int, default=4,
                     help="host threads that prepare served cases")
+    ap.add_argument("--serving-fp64-tiles", default="",
+                    help="tiles of the served FP64 2D Euler classes, order:kernel:lanes,... "
+                         "(kernel 1 point split, 2 variable split); default: each case's "
+                         "autotuned tile")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -634,8 +643,11 @@ def main(argv=None) -> int:
             f"{result['precision_error']['density_linf']:.3e}")
     if args.serving_requests > 0:
         del eng
+        tiles = {int(o): (int(k), int(n)) for o, k, n in
+                 (t.split(":") for t in args.serving_fp64_tiles.split(",") if t)}
         result["serving"] = serving_measurement(args.serving_requests, args.serving_seed,
-                                                args.serving_steps, args.serving_threads)
+                                                args.serving_steps, args.serving_threads,
+                                                tiles or None)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
         result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)

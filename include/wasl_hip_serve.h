@@ -49,6 +49,14 @@ typedef struct {
   int32_t setup_threads;    /* host threads preparing cases (default 4, at most 16) */
   int32_t fp64_max_order;   /* 2D Euler orders up to this run the FP64 stage (default 1; -1 none) */
   int32_t fp64_limited;     /* 2D Euler cases with limiting run the FP64 stage (default 1) */
+  /* Tile of the FP64 2D Euler stage by order, {stage_kernel, lanes_per_element}
+   * as in wasl_hip_numerics, for served cases without limiting whose numerics
+   * leave the tile to the autotuner (limited cases keep the autotuned tile, so
+   * that their troubled-cell decisions are those of a single engine). A served class runs one launch over all its cases, so the
+   * tile that suits a large mesh can beat the one a small case's autotuner
+   * picks. {0, 0}: the engine's autotuned tile. Defaults: P0 {0, 0}, P1 {1, 1},
+   * P2 {1, 2}, P3 {2, 4}. */
+  int32_t fp64_tile[4][2];
 } wasl_hip_server_config;
 
 void wasl_hip_server_config_default(wasl_hip_server_config *config);

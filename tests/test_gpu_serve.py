@@ -104,9 +104,12 @@ def test_fp64_classes_match_single_engine(gpu, order, limited, fp64_max_order):
         m = gpu.Mesh.from_input(K.vortex_mesh(12))
         kw = {}
         ic, prm, cfl, end = gpu.IC_VORTEX, K.VORTEX, 0.15, 0.2
-    # the engine's autotuned tile: the server's FP64 classes take the tile the
-    # autotuner gives the same key
-    g = gpu.Engine(m, gpu.EULER, order, precision=gpu.FP64, wavespeed=1.0, **kw)
+    # unlimited FP64 classes run the server's tile for the order, limited
+    # ones the engine's autotuned tile
+    from wasl_amd.serve import default_fp64_tiles
+    kernel, lanes = (0, 0) if limited else default_fp64_tiles()[order]
+    g = gpu.Engine(m, gpu.EULER, order, precision=gpu.FP64, wavespeed=1.0, stage_kernel=kernel,
+                   lanes=lanes, **kw)
     g.project(ic, prm)
     t, s, _ = g.advance(cfl, end)
     (state, ts, ss), = _run(gpu, [dict(mesh=m, equation=gpu.EULER, order=order, ic=ic,

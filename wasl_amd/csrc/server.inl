@@ -568,6 +568,13 @@ void wasl_hip_server_config_default(wasl_hip_server_config *c) {
   c->setup_threads = 4;
   c->fp64_max_order = 1;
   c->fp64_limited = 1;
+  // fixed tiles beat each case's autotuned one on the 64-request serving mix
+  // (P1 point split 1 lane, P2 point split 2 lanes, P3 variable split 4 lanes)
+  const int32_t tiles[4][2] = {{0, 0}, {1, 1}, {1, 2}, {2, 4}};
+  for (int p = 0; p < 4; ++p) {
+    c->fp64_tile[p][0] = tiles[p][0];
+    c->fp64_tile[p][1] = tiles[p][1];
+  }
 }
 
 int wasl_hip_server_create(const wasl_hip_server_config *config, wasl_hip_server **out) {
@@ -631,6 +638,14 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
                       r->num.polynomial_order <= s->cfg.fp64_max_order ||
                       (s->cfg.fp64_limited && r->num.limiting);
     r->num.precision = fp64 ? WASL_HIP_FP64 : s->cfg.precision;
+    const int order = r->num.polynomial_order;
+    // Limited cases keep the autotuned tile: troubled-cell decisions near
+    // vacuum follow rounding, and the single engine's tile reproduces them.
+    if (fp64 && euler && !r->num.limiting && r->num.stage_kernel == 0 && order >= 0 &&
+        order <= 3 && s->cfg.fp64_tile[order][0] != 0) {
+      r->num.stage_kernel = s->cfg.fp64_tile[order][0];
+      r->num.lanes_per_element = s->cfg.fp64_tile[order][1];
+    }
     r->ic = ic;
     r->params.assign(params, params + std::max<int64_t>(param_count, 0));
     r->cfl = cfl;

@@ -27,6 +27,7 @@ class ServerConfig(C.Structure):
         ("setup_threads", C.c_int32),
         ("fp64_max_order", C.c_int32),
         ("fp64_limited", C.c_int32),
+        ("fp64_tile", C.c_int32 * 8),
     ]
 
 
@@ -59,6 +60,15 @@ def _lib() -> C.CDLL:
     return lib
 
 
+def default_fp64_tiles() -> dict:
+    """{order: (stage_kernel, lanes_per_element)} of the served FP64 2D Euler
+    classes in the library's default configuration; (0, 0) is the engine's
+    autotuned tile."""
+    cfg = ServerConfig()
+    _lib().wasl_hip_server_config_default(C.byref(cfg))
+    return {p: (cfg.fp64_tile[2 * p], cfg.fp64_tile[2 * p + 1]) for p in range(4)}
+
+
 class Server:
     """A serving engine. submit() queues a case; pump() runs one scheduling
     round and returns the ids finished in it; result() hands back a finished
@@ -66,7 +76,7 @@ class Server:
 
     def __init__(self, precision: int = MIXED, max_resident: int = 256, memory_budget: int = 0,
                  steps_per_batch: int = 16, use_graphs: bool = True, setup_threads: int = 4,
-                 fp64_max_order: int = 1, fp64_limited: bool = True):
+                 fp64_max_order: int = 1, fp64_limited: bool = True, fp64_tiles=None):
         lib = _lib()
         cfg = ServerConfig()
         lib.wasl_hip_server_config_default(C.byref(cfg))
@@ -78,6 +88,10 @@ class Server:
         cfg.setup_threads = setup_threads
         cfg.fp64_max_order = fp64_max_order
         cfg.fp64_limited = int(bool(fp64_limited))
+        # {order: (stage_kernel, lanes_per_element)} for FP64 2D Euler classes
+        for order, (kernel, lanes) in (fp64_tiles or {}).items():
+            cfg.fp64_tile[2 * order] = kernel
+            cfg.fp64_tile[2 * order + 1] = lanes
         h = _P()
         _check(lib.wasl_hip_server_create(C.byref(cfg), C.byref(h)))
         self.h = h.value
