@@ -613,7 +613,10 @@ def main(argv=None) -> int:
         "config": {
             "workload": f"2D Euler isentropic vortex, P{p} modal Dubiner, "
                         f"2x{args.cells}x{args.cells}={E} jittered periodic triangles on [0,10]^2, "
-                        f"HLLC, SSP-RK3, CFL {cfl}, device dt, plain FP64",
+                        f"HLLC, SSP-RK3, CFL {cfl}, device dt, " +
+                        {"fp64": "plain FP64",
+                         "mixed": "mixed precision (FP32 matrix-core contractions, FP64 state)",
+                         "bf16": "BF16 volume contractions (ablation)"}[args.precision],
             "elements": E, "modes": Np, "variables": V,
             "dofs": E * Np * V, "lanes_per_element": eng.lanes,
             "stage_kernel": ("matrix-core mixed" if args.precision != "fp64" else
