@@ -372,7 +372,8 @@ def sod_run(args) -> int:
 
 
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
-                        setup_threads: int = 4, fp64_tiles=None) -> dict:
+                        setup_threads: int = 4, fp64_tiles=None,
+                        fp64_max_order: int = 1) -> dict:
     """The serving mix run plain (FP64, one case at a time) and served
     (mixed precision, continuous batching): aggregate DOF-updates/s of each,
     and time to solution under Poisson arrivals at 80 % of the served
@@ -402,7 +403,8 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
 
     done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles)
+                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles,
+                                          fp64_max_order=fp64_max_order)
     served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
     served_rate = served_updates / makespan
     capacity = count / makespan
@@ -414,7 +416,8 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
     done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals,
                                           steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles)
+                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles,
+                                          fp64_max_order=fp64_max_order)
     tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
     plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
 
@@ -498,6 +501,8 @@ def main(argv=None) -> int:
                     help="SSP-RK3 steps per scheduling round of the server (even)")
     ap.add_argument("--serving-threads", type=int, default=4,
                     help="host threads that prepare served cases")
+    ap.add_argument("--serving-fp64-max-order", type=int, default=1,
+                    help="served unlimited 2D Euler cases up to this order run the FP64 stage")
     ap.add_argument("--serving-fp64-tiles", default="",
                     help="tiles of the served FP64 2D Euler classes, order:kernel:lanes,... "
                          "(kernel 1 point split, 2 variable split); default: each case's "
@@ -650,7 +655,7 @@ def main(argv=None) -> int:
                  (t.split(":") for t in args.serving_fp64_tiles.split(",") if t)}
         result["serving"] = serving_measurement(args.serving_requests, args.serving_seed,
                                                 args.serving_steps, args.serving_threads,
-                                                tiles or None)
+                                                tiles or None, args.serving_fp64_max_order)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
         result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)

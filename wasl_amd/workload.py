@@ -139,7 +139,7 @@ def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True) 
 
 def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=W.MIXED,
                steps_per_batch: int = 16, max_resident: int = 256, setup_threads: int = 4,
-               fp64_tiles=None):
+               fp64_tiles=None, fp64_max_order: int = 1):
     """Submit each request at its arrival offset (all at once when arrivals is
     None) and pump the server until every request is finished. Returns
     (per-request (arrival, completion, steps, state) dict, makespan seconds,
@@ -149,7 +149,8 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
     from .serve import Server
 
     s = Server(precision=precision, steps_per_batch=steps_per_batch, max_resident=max_resident,
-               setup_threads=setup_threads, fp64_tiles=fp64_tiles)
+               setup_threads=setup_threads, fp64_tiles=fp64_tiles,
+               fp64_max_order=fp64_max_order)
     order = sorted(range(len(reqs)), key=lambda k: 0.0 if arrivals is None else arrivals[k])
     pending = list(order)
     ids = {}
