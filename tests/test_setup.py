@@ -162,3 +162,17 @@ def test_numerics_defaults(wasl):
     assert n.pressure_floor == 1e-10 and n.heat_capacity_ratio == 1.4
     assert n.velocity_y == 0.0
     assert ctypes.sizeof(n) == 104   # sizeof(wasl_hip_numerics) on x86-64
+
+
+def test_server_config_defaults(wasl):
+    """wasl_hip_server_config_default: the served FP64 tiles by order (P0 left
+    to the autotuner; P1 point split 1 lane, P2 point split 2 lanes, P3
+    variable split 4 lanes) and the struct layout the Python mirror assumes."""
+    from wasl_amd.serve import ServerConfig, default_fp64_tiles
+
+    assert default_fp64_tiles() == {0: (0, 0), 1: (1, 1), 2: (1, 2), 3: (2, 4)}
+    c = ServerConfig()
+    wasl.library().wasl_hip_server_config_default(ctypes.byref(c))
+    assert (c.max_resident, c.steps_per_batch, c.setup_threads) == (256, 16, 4)
+    assert (c.fp64_max_order, c.fp64_limited, c.use_graphs) == (1, 1, 1)
+    assert ctypes.sizeof(c) == 72   # sizeof(wasl_hip_server_config) on x86-64
