@@ -1215,8 +1215,10 @@ __device__ __forceinline__ SideF side_of_f(float gamma, const F4 &s, float nx, f
   o.v = s.my * o.inv_r;
   o.p = (gamma - 1.0f) * (s.E - 0.5f * (s.mx * o.u + s.my * o.v));
   o.un = o.u * nx + o.v * ny;
-  // FP32 rounding can push a positive pressure just below zero near vacuum
-  o.c = sqrtf(fmaxf(gamma * o.p * o.inv_r, 0.0f));
+  // FP32 rounding can push a positive pressure just below zero near vacuum.
+  // The hardware square root (within 1 ulp) is enough for a wave-speed bound
+  // and skips the correctly rounded expansion (denormal scaling, corrections).
+  o.c = __builtin_amdgcn_sqrtf(fmaxf(gamma * o.p * o.inv_r, 0.0f));
   return o;
 }
 
