@@ -248,11 +248,17 @@ class Case:
         self._check(self.lib.orc_residual(self.h, U, out))
         return out
 
-    def limit(self, U, threshold, density_floor=1e-10, pressure_floor=1e-10):
+    def limit(self, U, threshold, density_floor=1e-10, pressure_floor=1e-10, partial=False):
+        """Limited state and troubled flags. With partial=True a failure
+        (a cell average below a floor) returns (state, flags, message): the
+        state holds the elements limited before the reference threw, in
+        ascending element order (euler_modal_limiter.cpp:81)."""
         U = np.array(U, np.float64).ravel()
         flags = np.zeros(self.E, np.uint8)
-        self._check(self.lib.orc_limit(self.h, threshold, density_floor, pressure_floor, U,
-                                       flags))
+        rc = self.lib.orc_limit(self.h, threshold, density_floor, pressure_floor, U, flags)
+        if partial:
+            return U, flags, (self.lib.orc_error().decode() if rc != 0 else None)
+        self._check(rc)
         return U, flags
 
     def stable_dt(self, U, cfl):

@@ -561,7 +561,16 @@ int orc_limit(const orc_case *c, double threshold, double density_floor, double 
     std::vector<double> U(state, state + 4 * c->n * c->E());
     if (flags)
       sensor_flags(*c, U, threshold, flags);
-    wasl::limit_euler_modal(c->stencil, c->equation, threshold, density_floor, pressure_floor, U);
+    // limit_euler_modal works in place and may throw part way through (a cell
+    // average below a floor): the elements limited before the throw are
+    // handed back as well, as the restatement's in-place loop does.
+    try {
+      wasl::limit_euler_modal(c->stencil, c->equation, threshold, density_floor, pressure_floor,
+                              U);
+    } catch (...) {
+      std::copy(U.begin(), U.end(), state);
+      throw;
+    }
     std::copy(U.begin(), U.end(), state);
   });
 }

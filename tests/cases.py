@@ -14,6 +14,7 @@ import numpy as np
 from oracle import oracle as O
 from wasl_amd import meshgen as MG
 
+GAMMA = 1.4
 STEP_TOL = 1e-12
 RUN_TOL = 1e-10
 
@@ -94,3 +95,67 @@ def engine_tri(wasl, m, order, boundary=0, limiting=False, threshold=1e-3, floor
     return wasl.Engine(mesh, wasl.EULER, order, boundary=(boundary, boundary), limiting=limiting,
                        sensor_threshold=threshold, density_floor=floors, pressure_floor=floors,
                        lanes=lanes, precision=precision, stage_kernel=stage_kernel)
+
+
+# Positivity-engaged limiter inputs. A low-density, low-pressure background
+# (rho 0.05, u 1, v 0.5, p 0.01) with seeded random high modes of amplitude
+# POSITIVITY_AMPLITUDE[p] x |mean| on every variable: the characteristic WENO
+# blends oscillatory candidates, so most troubled cells reach the Zhang-Shu
+# limiter with density or pressure below the floor at some check points
+# (euler_positivity.hpp:63-102), some with density only scaled, some with the
+# pressure bisection as well, some untouched.
+POSITIVITY_BACKGROUND = (0, 0, 0, 0, 0, 0, 0.05, 1.0, 0.5, 0.01)
+POSITIVITY_AMPLITUDE = {1: 0.3, 2: 0.15, 3: 0.08}
+POSITIVITY_FLOORS = 1e-13
+
+
+def positivity_state(c, order, seed):
+    U = c.project(O.IC_EULER_REGIONS, POSITIVITY_BACKGROUND).reshape(4, c.Np, c.E)
+    rng = np.random.default_rng(seed)
+    U[:, 1:, :] = (POSITIVITY_AMPLITUDE[order] * np.abs(U[:, :1, :]) *
+                   rng.standard_normal((4, c.Np - 1, c.E)))
+    return U.ravel()
+
+
+def positivity_scales(c, U, threshold, density_floor, pressure_floor, routines):
+    """Per element, for the troubled cells of one limiter call: whether the
+    characteristic-WENO output had a check-point density below the floor
+    (the density pass scales), and the total high-mode scale the positivity
+    routine applied (1 where it left the cell alone). The call is recomposed
+    from the reference's single routines -- candidates T_k U_nb with the
+    target's own mean (euler_modal_limiter.cpp:43-66, summed in the same
+    order), euler_characteristic_weno, limit_euler_positivity -- and every
+    recomposed element must equal the case-level limiter's output bit for
+    bit. Returns (flags, density_engaged[E], scale[E], limited state)."""
+    nb, tr, nrm, cb = c.stencil()
+    E, Np = c.E, c.Np
+    Ul, flags = c.limit(U, threshold, density_floor, pressure_floor)
+    Uv = np.asarray(U, np.float64).reshape(4, Np, E)
+    Lv = Ul.reshape(4, Np, E)
+    dens = np.zeros(E, bool)
+    scale = np.ones(E)
+    for e in np.nonzero(flags)[0]:
+        t = Uv[:, :, e].copy()
+        cands = []
+        for k in range(nb.shape[1]):
+            n = int(nb[e, k])
+            if n < 0:
+                continue
+            cv = np.zeros((4, Np))
+            for v in range(4):
+                for tm in range(Np):
+                    x = 0.0
+                    for sm in range(Np):
+                        x += tr[e, k, tm, sm] * Uv[v, sm, n]
+                    cv[v, tm] = x
+                cv[v, 0] = t[v, 0]
+            cands.append(cv)
+        avg = t[:, 0] / np.sqrt(2.0)
+        lim = routines.characteristic_weno(GAMMA, avg, nrm[e, 0], nrm[e, 1], t,
+                                           np.array(cands) if cands else np.zeros(1),
+                                           len(cands), Np).reshape(4, Np)
+        dens[e] = bool(np.any(cb @ lim[0] < density_floor))
+        out, s = routines.positivity(GAMMA, lim, Np, cb, density_floor, pressure_floor)
+        scale[e] = s
+        assert np.array_equal(out.reshape(4, Np), Lv[:, :, e]), f"element {e}"
+    return flags, dens, scale, Ul

@@ -128,7 +128,66 @@ def vectors() -> None:
             for _ in range(3):
                 traj.append(c.step(traj[-1], c.stable_dt(traj[-1], 0.2), True, 1e-3))
             out[f"euler1d_p{p}_{tag}_steps"] = np.stack(traj)
+    positivity_vectors(out)
     np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
+
+
+def positivity_meshes():
+    import cases as K
+    from oracle import oracle as O
+
+    return (("transmissive", O.OUTFLOW, K.rarefaction_mesh(8)),
+            ("periodic", O.PERIODIC, K.vortex_mesh(8)))
+
+
+# The floor-error input: positivity_state at P2 on the transmissive mesh with
+# element FLOOR_ERROR_ELEMENTS[0]'s density mean and element [1]'s pressure
+# mean pushed below the floors. The reference throws at the first of them.
+FLOOR_ERROR_ELEMENTS = (37, 90)
+
+
+def floor_error_state(c):
+    import cases as K
+
+    U = K.positivity_state(c, 2, 42).reshape(4, c.Np, c.E)
+    s2 = np.sqrt(2.0)
+    a, b = FLOOR_ERROR_ELEMENTS
+    U[0, 0, a] = 0.5 * K.POSITIVITY_FLOORS * s2
+    rho, mx, my = U[0, 0, b] / s2, U[1, 0, b] / s2, U[2, 0, b] / s2
+    U[3, 0, b] = (0.5 * (mx * mx + my * my) / rho - 1e-3) * s2   # pressure -4e-4
+    return U.ravel()
+
+
+def positivity_vectors(out) -> None:
+    """Limiter inputs on which the Zhang-Shu positivity limiter acts
+    (tests/cases.py positivity_state): per order and boundary the input, the
+    reference's limited state and flags at threshold 1e-3, and two limited
+    SSP-RK3 steps at threshold 1e-10 (every cell troubled) from the limited
+    state; and the floor-error input with the reference's partial state."""
+    import cases as K
+
+    F = K.POSITIVITY_FLOORS
+    for tag, bnd, mesh in positivity_meshes():
+        for p in (1, 2, 3):
+            c = K.oracle_tri(mesh, p, boundary=bnd, kind="reference")
+            U = K.positivity_state(c, p, 40 + p)
+            Ul, flags = c.limit(U, 1e-3, F, F)
+            key = f"positivity_{tag}_p{p}"
+            out[key + "_state"] = U
+            out[key + "_limited"] = Ul
+            out[key + "_flags"] = flags
+            U0, _ = c.limit(U, 1e-10, F, F)
+            traj = [U0]
+            for _ in range(2):
+                traj.append(c.step(traj[-1], c.stable_dt(traj[-1], 0.1), True, 1e-10, F, F))
+            out[key + "_steps"] = np.stack(traj)
+    _, bnd, mesh = positivity_meshes()[0]
+    c = K.oracle_tri(mesh, 2, boundary=bnd, kind="reference")
+    U = floor_error_state(c)
+    part, flags, msg = c.limit(U, 1e-3, F, F, partial=True)
+    assert msg and "below a floor" in msg, msg
+    out["floor_error_p2_state"] = U
+    out["floor_error_p2_partial"] = part
 
 
 if __name__ == "__main__":

@@ -610,19 +610,22 @@ void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip:
                     int parity) {
   const whip::LimiterDev L = g.limiter();
   const dim3 grid(whip::limit_blocks(g.E)), grid256(whip::kScatterBlocks);
+  // Outside the stepping loop, err[1] holds the first element whose average
+  // is below a floor (INT_MAX when none); the scatter stops there.
+  const int *fail = ctl ? nullptr : g.err.as<int>() + 1;
   if (g.dim == 1) {
     hipLaunchKernelGGL((whip::limit_troubled_line<P>), grid, dim3(64), 0, g.stream, L, U,
                        g.side.as<double>(), g.list.as<int>(), count, ctl, parity,
                        g.err.as<int>());
     hipLaunchKernelGGL((whip::scatter_limited_line<P>), grid256, dim3(256), 0, g.stream, U, g.E,
-                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity);
+                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity, fail);
     return;
   }
   hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, g.stream, L, U,
                      g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.err.as<int>());
   hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, g.stream, U, g.E,
                      g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.traces_of(U),
-                     g.mxtable.as<float>());
+                     g.mxtable.as<float>(), fail);
 }
 
 // Limit the troubled elements listed by stage `slot` (counter index).
@@ -710,14 +713,24 @@ double stage_time_ms(wasl_hip_engine &g, int pairs) {
   return total;
 }
 
-void raise_device_error(wasl_hip_engine &g) {
-  int code = 0;
-  HIPCHECK(hipMemcpyAsync(&code, g.err.p, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+// The engine's error word: err[0] the device error code, err[1] the first
+// element (ascending) that the standalone limiter left unlimited because its
+// cell average is below a floor, INT_MAX when none.
+void reset_device_error(wasl_hip_engine &g) {
+  const int clear[2] = {0, INT_MAX};
+  HIPCHECK(hipMemcpyAsync(g.err.p, clear, sizeof clear, hipMemcpyHostToDevice, g.stream));
   HIPCHECK(hipStreamSynchronize(g.stream));
-  if (code != 0) {
-    HIPCHECK(hipMemsetAsync(g.err.p, 0, sizeof(int), g.stream));
+}
+
+void raise_device_error(wasl_hip_engine &g) {
+  int code[2] = {0, INT_MAX};
+  HIPCHECK(hipMemcpyAsync(code, g.err.p, sizeof code, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  if (code[0] != 0 || code[1] != INT_MAX) {
+    reset_device_error(g);
     HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
-    throw SimulationError(device_error_text(code));
+    HIPCHECK(hipStreamSynchronize(g.stream));
+    throw SimulationError(device_error_text(code[0]));
   }
 }
 
@@ -1155,8 +1168,8 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   HIPCHECK(hipMemsetAsync(g.u.p, 0, n, g.stream));
   g.counters.alloc(4 * sizeof(int), tot);
   HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
-  g.err.alloc(sizeof(int), tot);
-  HIPCHECK(hipMemsetAsync(g.err.p, 0, sizeof(int), g.stream));
+  g.err.alloc(2 * sizeof(int), tot);
+  reset_device_error(g);
   g.ctl.alloc(sizeof(whip::RunCtl), tot);
   g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
   g.result.alloc(sizeof(double), tot);

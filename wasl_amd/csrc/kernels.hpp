@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <climits>
 
 namespace whip {
 
@@ -2579,8 +2580,18 @@ __device__ __forceinline__ void limit_wave_core(const LimiterDev L, const double
     const double is2 = 1.0 / sqrt(2.0);
     const double a[4] = {o[0][0] * is2, o[1][0] * is2, o[2][0] * is2, o[3][0] * is2};
     if (a[0] <= L.density_floor || pressure_of(g, a[0], a[1], a[2], a[3]) <= L.pressure_floor) {
-      if (lane == 0)
+      // The reference throws here (euler_positivity.hpp:74-75): this element
+      // keeps its unlimited coefficients, and outside the stepping loop the
+      // smallest such element is recorded in err[1] so the scatter leaves it
+      // and every later element (ascending order, euler_modal_limiter.cpp:81)
+      // as they were.
+      if (vm)
+        side[static_cast<size_t>(idx) * 4 * Np + v * Np + m] = t[v][m];
+      if (lane == 0) {
         raise_error(ctl, err, kErrPositivity);
+        if (!ctl && err)
+          atomicMin(err + 1, e);
+      }
       __syncthreads();
       continue;
     }
@@ -2648,12 +2659,15 @@ __device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double
 // precision) also the FP32 edge traces of the limited element.
 template <int Np>
 __device__ __forceinline__ bool scatter_core(double *U, int E, const double *side, const int *list,
-                                             const int *count, RunCtl *ctl, int parity, int idx) {
+                                             const int *count, RunCtl *ctl, int parity, int idx,
+                                             const int *fail = nullptr) {
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return false;
   if (idx >= *count)
     return false;
   const int e = list[idx];
+  if (fail && e >= *fail)
+    return false;   // at or after the first element whose average is below a floor
   const double *src = side + static_cast<size_t>(idx) * 4 * Np;
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -2666,9 +2680,10 @@ __device__ __forceinline__ bool scatter_core(double *U, int E, const double *sid
 template <int P>
 __device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
                                              const int *count, RunCtl *ctl, int parity, int idx,
-                                             float *tr = nullptr, const float *mxtab = nullptr) {
+                                             float *tr = nullptr, const float *mxtab = nullptr,
+                                             const int *fail = nullptr) {
   constexpr int Np = Tri<P>::Np;
-  if (!scatter_core<Np>(U, E, side, list, count, ctl, parity, idx))
+  if (!scatter_core<Np>(U, E, side, list, count, ctl, parity, idx, fail))
     return;
   const int e = list[idx];
   const double *src = side + static_cast<size_t>(idx) * 4 * Np;
@@ -2709,14 +2724,17 @@ constexpr int kScatterBlocks = 32;
 template <int Np>
 __device__ __forceinline__ void scatter_strided(double *U, int E, const double *side,
                                                 const int *list, const int *count, RunCtl *ctl,
-                                                int parity) {
+                                                int parity, const int *fail) {
   if (ctl && (ctl->err || !ctl->s[parity].active))
     return;
   const int n = *count * 4 * Np;
+  const int first_bad = fail ? *fail : INT_MAX;
   const int stride = static_cast<int>(gridDim.x * blockDim.x);
   for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
     const int slot = i / (4 * Np), r = i - slot * 4 * Np;
-    U[static_cast<size_t>(r) * static_cast<unsigned>(E) + list[slot]] = side[i];
+    const int e = list[slot];
+    if (e < first_bad)
+      U[static_cast<size_t>(r) * static_cast<unsigned>(E) + e] = side[i];
   }
 }
 
@@ -2724,16 +2742,16 @@ template <int P>
 __global__ __launch_bounds__(256) void scatter_limited(double *U, int E, const double *side,
                                                        const int *list, const int *count,
                                                        RunCtl *ctl, int parity, float *tr,
-                                                       const float *mxtab) {
+                                                       const float *mxtab, const int *fail) {
   if (!tr) {
-    scatter_strided<Tri<P>::Np>(U, E, side, list, count, ctl, parity);
+    scatter_strided<Tri<P>::Np>(U, E, side, list, count, ctl, parity, fail);
     return;
   }
   // mixed precision: one element per thread (its FP32 traces are refreshed too)
   const int stride = static_cast<int>(gridDim.x * blockDim.x);
   for (int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); idx < *count;
        idx += stride)
-    scatter_body<P>(U, E, side, list, count, ctl, parity, idx, tr, mxtab);
+    scatter_body<P>(U, E, side, list, count, ctl, parity, idx, tr, mxtab, fail);
 }
 
 // Lines: two neighbours, (P+1) Gauss points and the two ends as check points.
@@ -2749,8 +2767,9 @@ __global__ __launch_bounds__(64) void limit_troubled_line(LimiterDev L, const do
 template <int P>
 __global__ __launch_bounds__(256) void scatter_limited_line(double *U, int E, const double *side,
                                                             const int *list, const int *count,
-                                                            RunCtl *ctl, int parity) {
-  scatter_strided<P + 1>(U, E, side, list, count, ctl, parity);
+                                                            RunCtl *ctl, int parity,
+                                                            const int *fail) {
+  scatter_strided<P + 1>(U, E, side, list, count, ctl, parity, fail);
 }
 
 // Batched limiter: tiles of 64 troubled-list slots per (case, chunk); the
