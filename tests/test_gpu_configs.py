@@ -8,8 +8,9 @@
             per-step parity (1e-12) and a 20-step device loop against the
             oracle's runner (1e-10 at the end, history rows);
   config 3  limited radial rarefaction, P2, 2x256x256 = 131072 triangles,
-            floors 1e-13: from the device state at t = 0.25 (limiter active),
-            the limiter and two limited steps against the oracle, troubled
+            floors 1e-13: from the device state where its limiter flags the
+            most cells (t in 0.005-0.1), the limiter and two limited steps
+            against the oracle, troubled
             flags bit-exact at every step boundary and the state within
             1e-12; then the device run to T = 0.3 with physical cell averages;
   config 4  P3 vortex on 2x512x512 = 524288 triangles: 40 steps mixed against
