@@ -208,6 +208,29 @@ int wasl_hip_residual(wasl_hip_engine *engine, const double *state, double *rhs)
  * decision (may be NULL). */
 int wasl_hip_limit(wasl_hip_engine *engine, uint8_t *flags);
 
+/* wasl_hip_limit on a caller's DEVICE buffer, allocating nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate or
+ * synchronise (code under HIP graph capture, tight serving loops that drive
+ * their own stages), e.g. as the post_stage of a caller's own RK loop.
+ *   state   device U[variable][mode][element] of this engine's mesh, limited
+ *           in place with the same kernels and results as wasl_hip_limit;
+ *   flags   device uint8 [E] receiving the sensor decision, or NULL;
+ *   status  device int receiving WASL_HIP_OK, or WASL_HIP_SIMULATION when a
+ *           cell average is below a floor (the state then holds the elements
+ *           before the first such element limited, as the reference leaves
+ *           it), or NULL;
+ *   stream  hipStream_t the work is enqueued on (NULL: the engine's stream).
+ * The call only enqueues kernels: it calls no hipMalloc/hipFree and does not
+ * wait. Its working memory is one static 64 MiB device array per GPU
+ * (error word, troubled list, staged values: 256 + 4 E + 32 Np E bytes,
+ * e.g. at most about 205,000 elements at P3 or 340,000 at P2); larger meshes
+ * are rejected with an error. Every call on a GPU shares that array, so calls
+ * that overlap on one GPU (other streams, other threads) race; calls in one
+ * stream order are safe. The engine must have been created with limiting on.
+ */
+int wasl_hip_limit_noalloc(wasl_hip_engine *engine, double *state, uint8_t *flags, int *status,
+                           void *stream);
+
 /* euler_cfl_timestep / advection_cfl_timestep / triangle_scalar_cfl_timestep
  * (src/time/cfl_timestep.cpp:53-84) of the resident state. */
 int wasl_hip_stable_dt(wasl_hip_engine *engine, double cfl, double *dt);

@@ -121,6 +121,7 @@ struct wasl_hip_engine {
   int stage_kernel = 1;       // 1 point split, 2 variable split
   double hmin = 0.0;
   hipStream_t stream = nullptr;
+  int device = 0;             // the HIP device current when the engine was created
   size_t device_bytes = 0;
 
   whip::PeriodicTrace trace;      // host copy for export
@@ -605,47 +606,75 @@ void launch_traces(const wasl_hip_engine &g, const double *U, float *T, const fl
   HIPCHECK(hipGetLastError());
 }
 
+// Working memory of one limiter call: the limited values staged before the
+// scatter, the troubled list, its counters and the error word (err[0] code,
+// err[1] first element whose average is below a floor).
+struct LimitBuffers {
+  double *side;
+  int *list, *counters, *err;
+};
+
+LimitBuffers engine_buffers(const wasl_hip_engine &g) {
+  return {g.side.as<double>(), g.list.as<int>(), g.counters.as<int>(), g.err.as<int>()};
+}
+
 template <int P>
 void launch_limit_p(const wasl_hip_engine &g, double *U, const int *count, whip::RunCtl *ctl,
-                    int parity) {
+                    int parity, const LimitBuffers &b, hipStream_t stream) {
   const whip::LimiterDev L = g.limiter();
   const dim3 grid(whip::limit_blocks(g.E)), grid256(whip::kScatterBlocks);
   // Outside the stepping loop, err[1] holds the first element whose average
   // is below a floor (INT_MAX when none); the scatter stops there.
-  const int *fail = ctl ? nullptr : g.err.as<int>() + 1;
+  const int *fail = ctl ? nullptr : b.err + 1;
   if (g.dim == 1) {
-    hipLaunchKernelGGL((whip::limit_troubled_line<P>), grid, dim3(64), 0, g.stream, L, U,
-                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity,
-                       g.err.as<int>());
-    hipLaunchKernelGGL((whip::scatter_limited_line<P>), grid256, dim3(256), 0, g.stream, U, g.E,
-                       g.side.as<double>(), g.list.as<int>(), count, ctl, parity, fail);
+    hipLaunchKernelGGL((whip::limit_troubled_line<P>), grid, dim3(64), 0, stream, L, U, b.side,
+                       b.list, count, ctl, parity, b.err);
+    hipLaunchKernelGGL((whip::scatter_limited_line<P>), grid256, dim3(256), 0, stream, U, g.E,
+                       b.side, b.list, count, ctl, parity, fail);
     return;
   }
-  hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, g.stream, L, U,
-                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.err.as<int>());
-  hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, g.stream, U, g.E,
-                     g.side.as<double>(), g.list.as<int>(), count, ctl, parity, g.traces_of(U),
-                     g.mxtable.as<float>(), fail);
+  hipLaunchKernelGGL((whip::limit_troubled<P>), grid, dim3(64), 0, stream, L, U, b.side, b.list,
+                     count, ctl, parity, b.err);
+  hipLaunchKernelGGL((whip::scatter_limited<P>), grid256, dim3(256), 0, stream, U, g.E, b.side,
+                     b.list, count, ctl, parity, g.traces_of(U), g.mxtable.as<float>(), fail);
 }
 
-// Limit the troubled elements listed by stage `slot` (counter index).
-void launch_limit(const wasl_hip_engine &g, double *U, int slot, whip::RunCtl *ctl, int parity) {
-  const int *count = g.counters.as<int>() + slot;
+// Limit the troubled elements listed under counter `slot` of b.
+void launch_limit(const wasl_hip_engine &g, double *U, int slot, whip::RunCtl *ctl, int parity,
+                  const LimitBuffers &b, hipStream_t stream) {
+  const int *count = b.counters + slot;
   switch (g.p) {
   case 0: return;  // P0 is returned unchanged (euler_modal_limiter.cpp:73-74)
-  case 1: launch_limit_p<1>(g, U, count, ctl, parity); break;
-  case 2: launch_limit_p<2>(g, U, count, ctl, parity); break;
-  default: launch_limit_p<3>(g, U, count, ctl, parity); break;
+  case 1: launch_limit_p<1>(g, U, count, ctl, parity, b, stream); break;
+  case 2: launch_limit_p<2>(g, U, count, ctl, parity, b, stream); break;
+  default: launch_limit_p<3>(g, U, count, ctl, parity, b, stream); break;
   }
   HIPCHECK(hipGetLastError());
 }
 
-template <int P> void launch_sensor_p(const wasl_hip_engine &g, uint8_t *flags) {
+void launch_limit(const wasl_hip_engine &g, double *U, int slot, whip::RunCtl *ctl, int parity) {
+  launch_limit(g, U, slot, ctl, parity, engine_buffers(g), g.stream);
+}
+
+// Troubled-cell sensor of U into b's list (counter 3) and flags (may be null).
+template <int P>
+void launch_sensor_p(const wasl_hip_engine &g, const double *U, uint8_t *flags,
+                     const LimitBuffers &b, hipStream_t stream) {
   constexpr int kTriNp = (P + 1) * (P + 2) / 2;
   const auto kernel = g.dim == 1 ? whip::sensor<P + 1> : whip::sensor<kTriNp>;
-  hipLaunchKernelGGL(kernel, dim3((g.E + 255) / 256), dim3(256), 0, g.stream,
-                     g.u.as<double>(), g.E, g.num.sensor_threshold, g.stencil.highest_begin,
-                     flags, g.list.as<int>(), g.counters.as<int>() + 3);
+  hipLaunchKernelGGL(kernel, dim3((g.E + 255) / 256), dim3(256), 0, stream, U, g.E,
+                     g.num.sensor_threshold, g.stencil.highest_begin, flags, b.list,
+                     b.counters + 3);
+}
+
+void launch_sensor(const wasl_hip_engine &g, const double *U, uint8_t *flags,
+                   const LimitBuffers &b, hipStream_t stream) {
+  switch (g.p) {
+  case 1: launch_sensor_p<1>(g, U, flags, b, stream); break;
+  case 2: launch_sensor_p<2>(g, U, flags, b, stream); break;
+  default: launch_sensor_p<3>(g, U, flags, b, stream); break;
+  }
+  HIPCHECK(hipGetLastError());
 }
 
 bool limiting(const wasl_hip_engine &g) { return g.euler() && g.num.limiting && g.p > 0; }
@@ -938,6 +967,7 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
       num.precision != WASL_HIP_BF16)
     throw std::runtime_error("engine: precision must be FP64, MIXED or BF16");
   g.mesh = mesh;
+  HIPCHECK(hipGetDevice(&g.device));
   HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
   if (const char *env = std::getenv("WASL_HIP_GRAPHS"))
     g.use_graphs = std::atoi(env) != 0;
@@ -1177,6 +1207,84 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   if (g.euler() && g.dim == 2 && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
       num.lanes_per_element == 0)
     autotune(g);
+}
+
+}  // namespace
+
+// ------------------------------------------------ allocation-free limiter ----
+//
+// wasl_hip_limit_noalloc works on a caller's device buffer and keeps its
+// working memory (error word, troubled counter and list, staged limited
+// values) in this file-scope array instead of the engine's buffers, so the
+// call allocates nothing and, with device-side status, synchronises nothing.
+// Each device holds its own copy for the life of the process. Its device
+// address is looked up once per device with hipGetSymbolAddress; host code
+// only ever uses that address, never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;   // 64 MiB per device
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;   // per device index
+
+void *scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0)
+    throw std::runtime_error("limit_noalloc: invalid device");
+  if (static_cast<size_t>(device) >= g_scratch_ptr.size())
+    g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    int current = 0;
+    HIPCHECK(hipGetDevice(&current));
+    if (current != device)
+      HIPCHECK(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device)
+      (void)hipSetDevice(current);
+    check(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
+// Scratch layout of one call on E elements of Np modes (bytes from the base):
+// the error word and counters in the first 256 bytes, then list[E] int32,
+// then side[E][4][Np] FP64, each 256-byte aligned.
+struct ScratchLayout {
+  size_t list, side, total;
+};
+
+ScratchLayout scratch_layout(int E, int Np) {
+  const auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  ScratchLayout l;
+  l.list = 256;
+  l.side = up(l.list + sizeof(int) * static_cast<size_t>(E));
+  l.total = l.side + sizeof(double) * 4 * static_cast<size_t>(Np) * E;
+  return l;
+}
+
+// Clears the error word and troubled counter of a call: a kernel rather than
+// a memset, so that a caller may capture the call in a graph.
+__global__ void noalloc_begin(int *err, int *counters, uint8_t *flags, int E, int clear_flags) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i == 0) {
+    err[0] = 0;
+    err[1] = INT_MAX;
+    counters[3] = 0;
+  }
+  if (clear_flags && flags)
+    for (int e = i; e < E; e += static_cast<int>(gridDim.x * blockDim.x))
+      flags[e] = 0;
+}
+
+// The call's outcome: 0, or WASL_HIP_SIMULATION when a cell average was
+// below a floor (the reference's "Euler positivity limiter: cell average is
+// below a floor").
+__global__ void noalloc_end(const int *err, int *status) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *status = (err[0] != 0 || err[1] != INT_MAX) ? WASL_HIP_SIMULATION : WASL_HIP_OK;
 }
 
 }  // namespace
@@ -1579,16 +1687,50 @@ int wasl_hip_limit(wasl_hip_engine *g, uint8_t *flags) {
     if (!g->lnbr.p)
       throw std::runtime_error("engine: create the engine with limiting on to limit");
     HIPCHECK(hipMemsetAsync(g->counters.as<int>() + 3, 0, sizeof(int), g->stream));
-    switch (g->p) {
-    case 1: launch_sensor_p<1>(*g, g->flags.as<uint8_t>()); break;
-    case 2: launch_sensor_p<2>(*g, g->flags.as<uint8_t>()); break;
-    default: launch_sensor_p<3>(*g, g->flags.as<uint8_t>()); break;
-    }
+    launch_sensor(*g, g->u.as<double>(), g->flags.as<uint8_t>(), engine_buffers(*g), g->stream);
     launch_limit(*g, g->u.as<double>(), 3, nullptr, 0);
     if (flags)
       HIPCHECK(hipMemcpyAsync(flags, g->flags.p, g->E, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
     raise_device_error(*g);
+  });
+}
+
+int wasl_hip_limit_noalloc(wasl_hip_engine *g, double *state, uint8_t *flags, int *status,
+                           void *stream) {
+  return guarded([&] {
+    if (!g || !state)
+      throw std::runtime_error("limit_noalloc: null argument");
+    if (!g->euler())
+      throw std::runtime_error("engine: the limiter applies to 2D Euler");
+    if (g->p > 0 && !g->lnbr.p)
+      throw std::runtime_error("engine: create the engine with limiting on to limit");
+    const ScratchLayout l = scratch_layout(g->E, g->Np);
+    if (l.total > kScratchBytes)
+      throw std::runtime_error("limit_noalloc: " + std::to_string(g->E) + " elements of " +
+                               std::to_string(g->Np) + " modes need " +
+                               std::to_string(l.total) + " bytes of scratch, more than its " +
+                               std::to_string(kScratchBytes) + "-byte maximum");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->stream;
+    char *base = static_cast<char *>(scratch_on(g->device));
+    LimitBuffers b;
+    b.err = reinterpret_cast<int *>(base);
+    b.counters = reinterpret_cast<int *>(base + 16);
+    b.list = reinterpret_cast<int *>(base + l.list);
+    b.side = reinterpret_cast<double *>(base + l.side);
+    // P0 is returned unchanged (euler_modal_limiter.cpp:73-74) with no flag set
+    const int p0 = g->p == 0 ? 1 : 0;
+    hipLaunchKernelGGL(noalloc_begin, dim3(p0 ? (g->E + 255) / 256 : 1), dim3(256), 0, s, b.err,
+                       b.counters, flags, g->E, p0);
+    HIPCHECK(hipGetLastError());
+    if (!p0) {
+      launch_sensor(*g, state, flags, b, s);
+      launch_limit(*g, state, 3, nullptr, 0, b, s);
+    }
+    if (status) {
+      hipLaunchKernelGGL(noalloc_end, dim3(1), dim3(64), 0, s, b.err, status);
+      HIPCHECK(hipGetLastError());
+    }
   });
 }
 
