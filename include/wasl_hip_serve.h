@@ -25,7 +25,12 @@
  * FP64. Cases of one (dimension, order, stage precision) form a class;
  * classes run concurrently on streams of their own.
  *
- * Threading: one thread drives a server; the server owns its worker threads.
+ * Cases with limiting are limited once after their projection, as the
+ * reference's case drivers do (src/case/euler_triangle.cpp:37-41).
+ *
+ * Threading: the server owns its worker threads. Its calls may come from
+ * several threads; they are serialised (a submit made while another thread
+ * is inside wasl_hip_server_pump waits for that round to end).
  */
 #ifndef WASL_HIP_SERVE_H
 #define WASL_HIP_SERVE_H

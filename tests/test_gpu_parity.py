@@ -233,3 +233,32 @@ def test_line_boundaries(gpu, kinds):
                    boundary_values=(0.3, -0.2))
     U = o.project(O.IC_SCALAR_PULSE)
     assert K.rel_inf(g.residual(U), o.residual(U)) <= K.STEP_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_nan_fluxes_fail_like_the_reference(gpu, order):
+    """The unlimited projection of a disc of low density and pressure has
+    negative pressures at face points, so HLLC's sound speed is NaN there.
+    The reference's std::min/std::max keep their first operand when one is
+    NaN, the NaN reaches a cell average in the first step, and the next CFL
+    evaluation fails; the device's wave bounds follow the same rule and the
+    same failure. The coarse limited P1 rarefaction fails the same way."""
+    F = 1e-13
+    cases = [(K.rarefaction_mesh(10), O.IC_EULER_REGIONS, K.JUMP)]
+    if order == 1:
+        cases.append((K.rarefaction_mesh(8), O.IC_RAREFACTION, K.RAREFACTION))
+    for m, ic, prm in cases:
+        o = K.oracle_tri(m, order, boundary=O.OUTFLOW)
+        U = o.project(ic, prm)
+        dt = o.stable_dt(U, 0.1)
+        U1 = o.step(U, dt, True, 1e-3, F, F)
+        with pytest.raises(O.OracleError, match="Euler cell average is not physical"):
+            o.stable_dt(U1, 0.1)
+        g = K.engine_tri(gpu, m, order, boundary=gpu.OUTFLOW, limiting=True, floors=F)
+        g.state = U
+        g.step(dt)
+        with pytest.raises(gpu.SimulationError, match="Euler cell average is not physical"):
+            g.stable_time_step(0.1)
+        g.state = U
+        with pytest.raises(gpu.SimulationError, match="Euler cell average is not physical"):
+            g.advance(0.1, 0.05)

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 def _requests(w):
     """A small mix: Euler P1-P3 of uneven sizes (vortex, smooth field),
-    limited radial rarefactions and 1D advection. Every launch class holds
+    a limited radial rarefaction, limited low-density discs and 1D advection. Every launch class holds
     at least two cases, so each case's first tile (not the launch's) must
     lead its step control and clear its troubled-element count."""
     vm = {n: w.Mesh.from_input(K.vortex_mesh(n)) for n in (6, 9, 14)}
@@ -29,7 +29,10 @@ def _requests(w):
     rm8 = w.Mesh.from_input(K.rarefaction_mesh(8))
     lm = w.Mesh.from_input(MG.jittered_line(300))
     lm2 = w.Mesh.from_input(MG.jittered_line(170))
-    lim = dict(equation=w.EULER, ic=w.IC_RAREFACTION, params=K.RAREFACTION,
+    # limited discs of low density and pressure (troubled from the start; a
+    # P1 rarefaction on these coarse meshes loses a physical cell average in
+    # its first step, in the reference as well)
+    lim = dict(equation=w.EULER, ic=w.IC_EULER_REGIONS, params=K.JUMP,
                boundary=(w.OUTFLOW, w.OUTFLOW), limiting=True, cfl=0.1, density_floor=1e-13,
                pressure_floor=1e-13)
     return [
@@ -111,12 +114,50 @@ def test_fp64_classes_match_single_engine(gpu, order, limited, fp64_max_order):
     g = gpu.Engine(m, gpu.EULER, order, precision=gpu.FP64, wavespeed=1.0, stage_kernel=kernel,
                    lanes=lanes, **kw)
     g.project(ic, prm)
+    if limited:
+        g.limit()
     t, s, _ = g.advance(cfl, end)
     (state, ts, ss), = _run(gpu, [dict(mesh=m, equation=gpu.EULER, order=order, ic=ic,
                                        params=prm, end_time=end, cfl=cfl, **kw)],
                             fp64_max_order=fp64_max_order)
     assert (ts, ss) == (t, s)
     assert np.array_equal(state, g.state)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_served_limited_case_is_the_case_composition(gpu, order):
+    """A served limited case follows the reference's case driver: project,
+    limit the projected state, then step (euler_triangle.cpp:23-47). On the
+    disc of low density and pressure (cases.JUMP) the initial limit decides
+    the run: the served result equals the engine's project -> limit ->
+    advance bit for bit, while project -> advance loses a physical cell
+    average after its first step, on the device as in the oracle (the
+    projection's negative face pressures give NaN fluxes)."""
+    from oracle import oracle as O
+
+    mesh_in = K.rarefaction_mesh(10)
+    m = gpu.Mesh.from_input(mesh_in)
+    kw = dict(boundary=(gpu.OUTFLOW, gpu.OUTFLOW), limiting=True, density_floor=1e-13,
+              pressure_floor=1e-13)
+    cfl, end = 0.1, 0.01
+    g = gpu.Engine(m, gpu.EULER, order, **kw)
+    g.project(gpu.IC_EULER_REGIONS, K.JUMP)
+    assert g.limit().sum() > 0
+    t, s, _ = g.advance(cfl, end)
+    (state, ts, ss), = _run(gpu, [dict(mesh=m, equation=gpu.EULER, order=order,
+                                       ic=gpu.IC_EULER_REGIONS, params=K.JUMP, end_time=end,
+                                       cfl=cfl, **kw)])
+    assert (ts, ss) == (t, s)
+    assert np.array_equal(state, g.state)
+
+    o = K.oracle_tri(mesh_in, order, boundary=O.OUTFLOW)
+    with pytest.raises(O.OracleError, match="Euler cell average is not physical"):
+        o.run(o.project(O.IC_EULER_REGIONS, K.JUMP), cfl, end, limiting=True,
+              density_floor=1e-13, pressure_floor=1e-13)
+    h = gpu.Engine(m, gpu.EULER, order, **kw)
+    h.project(gpu.IC_EULER_REGIONS, K.JUMP)
+    with pytest.raises(gpu.SimulationError, match="^simulation: Euler cell average is not physical$"):
+        h.advance(cfl, end)
 
 
 def test_few_slots_admit_continuously(gpu):

@@ -6,7 +6,6 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <deque>
 #include <map>
 #include <memory>
 #include <set>
@@ -85,29 +84,78 @@ bool all_of_kind(const NamedBoundaries &boundaries, BoundaryKind kind) {
 
 // ------------------------------------------------------------- runner ----
 
-// Collects history and snapshots and rotates checkpoints
-// (src/case/serial_case_runner.cpp:47-104).
-class SerialOutput {
+// Output events at a step: a history row, a VTU snapshot, a checkpoint.
+enum OutputEvent : unsigned { kHistory = 1u, kSnapshot = 2u, kCheckpoint = 4u };
+
+// The output cadence of a run, fixed by its run control: every multiple of
+// an interval, and every kind of output at the final step (end_time or
+// max_steps), as src/case/serial_case_runner.cpp:132-141 emits them. The
+// device loop runs uninterrupted from one stop to the next.
+class OutputPlan {
 public:
-  SerialOutput(const CaseFiles &files, const CaseMesh &mesh, const SerialCaseModel &model,
-               std::uint64_t keep)
-      : files_(files), mesh_(mesh), model_(model), keep_(keep) {
-    std::filesystem::create_directories(files_.output_directory);
+  explicit OutputPlan(const RunControl &run) : run_(run) {}
+
+  bool running(const RunPosition &at) const {
+    return at.time < run_.end_time && at.step < run_.max_steps;
   }
 
-  void record_history(const RunPosition &at, double dt) {
-    history_.push_back({at.step, at.time, dt, model_.mass()});
+  // The first step after `step` at which some output falls due (or
+  // max_steps): where the device loop next hands control back.
+  std::uint64_t next_stop(std::uint64_t step) const {
+    const auto after = [step](std::uint64_t every) { return step - step % every + every; };
+    return std::min({run_.max_steps, after(run_.history_interval),
+                     after(run_.visualization_interval), after(run_.checkpoint_interval)});
   }
 
-  void write_snapshot(const RunPosition &at) {
-    const auto name = numbered_step_name("solution", at.step, ".vtu");
-    write_vtu(files_.output_directory / name, mesh_.prepared, model_.cell_averages(),
-              model_.variable_count);
-    snapshots_.emplace_back(at.time, name);
+  unsigned events(const RunPosition &at) const {
+    const bool last = at.time >= run_.end_time || at.step == run_.max_steps;
+    const auto due = [&](std::uint64_t every, unsigned bit) {
+      return (last || at.step % every == 0) ? bit : 0u;
+    };
+    return due(run_.history_interval, kHistory) | due(run_.visualization_interval, kSnapshot) |
+           due(run_.checkpoint_interval, kCheckpoint);
   }
 
-  void write_restart(const RunPosition &at) {
-    const auto path = files_.output_directory / numbered_step_name("checkpoint", at.step, ".h5");
+private:
+  const RunControl &run_;
+};
+
+// The files a run leaves in its output directory: history.csv and
+// solution.pvd written at the end from the rows and snapshots gathered, the
+// solution_<step>.vtu snapshots, and the newest keep checkpoint_<step>.h5
+// files, the older ones deleted as new ones arrive (a ring of keep slots).
+class CaseOutputs {
+public:
+  CaseOutputs(const CaseFiles &files, const CaseMesh &mesh, const SerialCaseModel &model,
+              std::uint64_t keep)
+      : dir_(files.output_directory), mesh_(mesh), model_(model), keep_(keep) {
+    std::filesystem::create_directories(dir_);
+  }
+
+  void emit(unsigned events, const RunPosition &at, double dt) {
+    if (events & kHistory)
+      rows_.push_back({at.step, at.time, dt, model_.mass()});
+    if (events & kSnapshot) {
+      const std::string vtu = numbered_step_name("solution", at.step, ".vtu");
+      write_vtu(dir_ / vtu, mesh_.prepared, model_.cell_averages(), model_.variable_count);
+      pvd_.emplace_back(at.time, vtu);
+    }
+    if (events & kCheckpoint)
+      checkpoint(at);
+  }
+
+  // The run's last checkpoint (one at the final position if the cadence
+  // wrote none), then the history and collection files.
+  std::filesystem::path close(const RunPosition &at) {
+    if (newest_.empty())
+      checkpoint(at);
+    write_csv_history(dir_ / "history.csv", rows_);
+    write_pvd(dir_ / "solution.pvd", pvd_);
+    return newest_;
+  }
+
+private:
+  void checkpoint(const RunPosition &at) {
     Checkpoint c;
     c.time = at.time;
     c.step = at.step;
@@ -116,37 +164,27 @@ public:
     c.element_count = mesh_.prepared.element_count();
     c.prepared_mesh = mesh_.path.string();
     c.coefficients = model_.to_checkpoint();
-    write_checkpoint(path, c);
-    checkpoints_.push_back(path);
-    if (checkpoints_.size() > keep_) {
-      std::filesystem::remove(checkpoints_.front());
-      checkpoints_.pop_front();
+    newest_ = dir_ / numbered_step_name("checkpoint", at.step, ".h5");
+    write_checkpoint(newest_, c);
+    if (ring_.size() < keep_) {
+      ring_.push_back(newest_);
+    } else {
+      std::filesystem::remove(ring_[oldest_]);
+      ring_[oldest_] = newest_;
+      oldest_ = (oldest_ + 1) % ring_.size();
     }
-    last_ = path;
   }
 
-  void finish() const {
-    write_csv_history(files_.output_directory / "history.csv", history_);
-    write_pvd(files_.output_directory / "solution.pvd", snapshots_);
-  }
-
-  const std::filesystem::path &last_checkpoint() const { return last_; }
-
-private:
-  const CaseFiles &files_;
+  std::filesystem::path dir_;
   const CaseMesh &mesh_;
   const SerialCaseModel &model_;
   std::uint64_t keep_;
-  std::vector<HistoryRow> history_;
-  std::vector<std::pair<double, std::string>> snapshots_;
-  std::deque<std::filesystem::path> checkpoints_;
-  std::filesystem::path last_;
+  std::vector<HistoryRow> rows_;
+  std::vector<std::pair<double, std::string>> pvd_;
+  std::vector<std::filesystem::path> ring_;   // checkpoints on disk
+  std::size_t oldest_ = 0;                    // ring slot written longest ago
+  std::filesystem::path newest_;
 };
-
-// First step after `step` that is a multiple of `interval`.
-std::uint64_t next_multiple(std::uint64_t step, std::uint64_t interval) {
-  return (step / interval + 1) * interval;
-}
 
 // A device engine behind the model's callbacks.
 struct DeviceCase {
@@ -282,9 +320,9 @@ CaseMesh load_case_mesh(const std::filesystem::path &prepared_mesh, int dimensio
   return mesh;
 }
 
-// src/case/serial_case_runner.cpp:117-149. The device takes every step between
-// two output events (history, snapshot, checkpoint or max_steps) without a
-// host round trip; outputs happen at the same steps as in the reference.
+// run_serial_case (src/case/serial_case_runner.cpp:117-149): the same outputs
+// at the same steps, but the device takes every step from one output stop of
+// the plan to the next without a host round trip.
 SimulationSummary run_serial_case(const CaseFiles &files, const RunControl &run,
                                   const CaseMesh &mesh, const SerialCaseModel &model) {
   RunPosition at{0, run.start_time};
@@ -296,32 +334,20 @@ SimulationSummary run_serial_case(const CaseFiles &files, const RunControl &run,
       at.step > run.max_steps)
     throw std::runtime_error("simulation restart: checkpoint is outside the configured run range");
 
-  SerialOutput output(files, mesh, model, run.keep_checkpoints);
-  output.record_history(at, 0.0);
-  output.write_snapshot(at);
-
-  while (at.time < run.end_time && at.step < run.max_steps) {
-    const std::uint64_t limit = std::min({run.max_steps, next_multiple(at.step, run.history_interval),
-                                          next_multiple(at.step, run.visualization_interval),
-                                          next_multiple(at.step, run.checkpoint_interval)});
-    const std::uint64_t before = at.step;
-    const double dt = model.advance_until(at, limit, run.end_time);
-    if (at.step == before)
+  const OutputPlan plan(run);
+  CaseOutputs outputs(files, mesh, model, run.keep_checkpoints);
+  outputs.emit(kHistory | kSnapshot, at, 0.0);
+  while (plan.running(at)) {
+    const std::uint64_t from = at.step;
+    const double dt = model.advance_until(at, plan.next_stop(at.step), run.end_time);
+    if (at.step == from)
       break;
-    const bool final = at.time >= run.end_time || at.step == run.max_steps;
-    if (at.step % run.history_interval == 0 || final)
-      output.record_history(at, dt);
-    if (at.step % run.visualization_interval == 0 || final)
-      output.write_snapshot(at);
-    if (at.step % run.checkpoint_interval == 0 || final)
-      output.write_restart(at);
+    outputs.emit(plan.events(at), at, dt);
   }
   if (at.time < run.end_time)
     throw std::runtime_error(kMaxStepsMessage);
-  if (output.last_checkpoint().empty())
-    output.write_restart(at);
-  output.finish();
-  return {at.step, at.time, output.last_checkpoint()};
+  const std::filesystem::path last = outputs.close(at);
+  return {at.step, at.time, last};
 }
 
 void validate_euler_case(const EulerCase &c) {

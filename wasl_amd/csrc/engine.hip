@@ -1209,6 +1209,27 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     autotune(g);
 }
 
+// limit_euler_modal on the resident state in place
+// (src/limiting/euler_modal_limiter.cpp:70-98); flags (host, may be null)
+// receive the sensor decision. Throws the reference's message on a floor
+// error.
+void limit_resident(wasl_hip_engine &g, uint8_t *flags) {
+  if (g.p == 0) {
+    if (flags)
+      std::fill(flags, flags + g.E, 0);
+    return;
+  }
+  if (!g.lnbr.p)
+    throw std::runtime_error("engine: create the engine with limiting on to limit");
+  HIPCHECK(hipMemsetAsync(g.counters.as<int>() + 3, 0, sizeof(int), g.stream));
+  launch_sensor(g, g.u.as<double>(), g.flags.as<uint8_t>(), engine_buffers(g), g.stream);
+  launch_limit(g, g.u.as<double>(), 3, nullptr, 0);
+  if (flags)
+    HIPCHECK(hipMemcpyAsync(flags, g.flags.p, g.E, hipMemcpyDeviceToHost, g.stream));
+  HIPCHECK(hipStreamSynchronize(g.stream));
+  raise_device_error(g);
+}
+
 }  // namespace
 
 // ------------------------------------------------ allocation-free limiter ----
@@ -1679,20 +1700,7 @@ int wasl_hip_limit(wasl_hip_engine *g, uint8_t *flags) {
   return guarded([&] {
     if (!g->euler())
       throw std::runtime_error("engine: the limiter applies to 2D Euler");
-    if (g->p == 0) {
-      if (flags)
-        std::fill(flags, flags + g->E, 0);
-      return;
-    }
-    if (!g->lnbr.p)
-      throw std::runtime_error("engine: create the engine with limiting on to limit");
-    HIPCHECK(hipMemsetAsync(g->counters.as<int>() + 3, 0, sizeof(int), g->stream));
-    launch_sensor(*g, g->u.as<double>(), g->flags.as<uint8_t>(), engine_buffers(*g), g->stream);
-    launch_limit(*g, g->u.as<double>(), 3, nullptr, 0);
-    if (flags)
-      HIPCHECK(hipMemcpyAsync(flags, g->flags.p, g->E, hipMemcpyDeviceToHost, g->stream));
-    HIPCHECK(hipStreamSynchronize(g->stream));
-    raise_device_error(*g);
+    limit_resident(*g, flags);
   });
 }
 

@@ -195,12 +195,18 @@ __device__ __forceinline__ C4 normal_flux(const C4 &s, const Side &w, double nx,
   return {s.r * w.un, s.mx * w.un + w.p * nx, s.my * w.un + w.p * ny, (s.E + w.p) * w.un};
 }
 
+// std::min / std::max as the reference calls them: with a NaN operand they
+// return the first one (fmin/fmax would return the other), so a NaN sound
+// speed (a negative point pressure) propagates exactly as it does there.
+__device__ __forceinline__ double first_min(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double first_max(double a, double b) { return a < b ? b : a; }
+
 // HLLC flux with Davis wave-speed bounds, evaluated (left, right, n_left)
 // (flux/hllc.hpp:40-74).
 __device__ inline C4 hllc(double gamma, const C4 &L, const C4 &R, double nx, double ny) {
   const Side wl = side_of(gamma, L, nx, ny), wr = side_of(gamma, R, nx, ny);
-  const double sl = fmin(wl.un - wl.c, wr.un - wr.c);
-  const double sr = fmax(wl.un + wl.c, wr.un + wr.c);
+  const double sl = first_min(wl.un - wl.c, wr.un - wr.c);
+  const double sr = first_max(wl.un + wl.c, wr.un + wr.c);
   if (sl >= 0.0)
     return normal_flux(L, wl, nx, ny);
   if (sr <= 0.0)

@@ -129,6 +129,10 @@ struct wasl_hip_server {
   int *errs_host = nullptr;               // pinned
   size_t slots_capacity = 0;
 
+  // Every C ABI call on the server holds api_mu for its duration, so calls
+  // from several threads are serialised (a submit from another thread waits
+  // for a running pump round). mu guards what the preparation workers share.
+  mutable std::mutex api_mu;
   // workers
   std::vector<std::thread> workers;
   std::mutex mu;
@@ -161,7 +165,8 @@ struct wasl_hip_server {
 
 namespace {
 
-// Worker: build the engine, project the initial condition, fix the end time.
+// Worker: build the engine, project the initial condition (and limit it when
+// the case limits, as euler_triangle.cpp:37-41 does), fix the end time.
 void prepare(Request &r) {
   try {
     auto g = std::make_unique<wasl_hip_engine>();
@@ -171,6 +176,8 @@ void prepare(Request &r) {
                       static_cast<int64_t>(r.params.size()));
     HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
                             g->stream));
+    if (limiting(*g))
+      limit_resident(*g, nullptr);
     if (r.end_steps > 0) {
       const double lambda = resident_speed(*g);
       const double stable = lambda > 0.0
@@ -624,6 +631,7 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
   return guarded([&] {
     if (!s || !mesh || !numerics || !id || (param_count > 0 && !params))
       throw std::runtime_error("server: null argument");
+    std::lock_guard<std::mutex> api(s->api_mu);
     if (!(cfl > 0.0) || (end_steps == 0 && !(end_time > 0.0)) || max_steps == 0)
       throw std::runtime_error("server: invalid run control");
     const bool euler = numerics->equation == WASL_HIP_EULER;
@@ -652,9 +660,9 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
     r->end_time = end_time;
     r->end_steps = end_steps;
     r->max_steps = max_steps;
+    std::lock_guard<std::mutex> lock(s->mu);
     r->id = s->next_id++;
     *id = r->id;
-    std::lock_guard<std::mutex> lock(s->mu);
     s->queue.push_back(r->id);
     s->requests[r->id] = std::move(r);
   });
@@ -665,6 +673,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
   return guarded([&] {
     if (!s)
       throw std::runtime_error("server: null argument");
+    std::lock_guard<std::mutex> api(s->api_mu);
     std::vector<uint64_t> done_now, ready_now;
     {
       std::unique_lock<std::mutex> lock(s->mu);
@@ -792,6 +801,8 @@ int wasl_hip_server_result(wasl_hip_server *s, uint64_t id, double *state, int64
   return guarded([&] {
     if (!s)
       throw std::runtime_error("server: null argument");
+    std::lock_guard<std::mutex> api(s->api_mu);
+    std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->requests.find(id);
     if (it == s->requests.end() || it->second->phase != Phase::done)
       throw std::runtime_error("server: request is not finished");
@@ -816,6 +827,7 @@ int wasl_hip_server_result(wasl_hip_server *s, uint64_t id, double *state, int64
 int64_t wasl_hip_server_state_size(const wasl_hip_server *s, uint64_t id) {
   if (!s)
     return -1;
+  std::lock_guard<std::mutex> api(s->api_mu);
   auto it = s->requests.find(id);
   if (it == s->requests.end())
     return -1;
@@ -830,6 +842,8 @@ int64_t wasl_hip_server_state_size(const wasl_hip_server *s, uint64_t id) {
 int64_t wasl_hip_server_count(const wasl_hip_server *s, int what) {
   if (!s)
     return -1;
+  std::lock_guard<std::mutex> api(s->api_mu);
+  std::lock_guard<std::mutex> lock(const_cast<wasl_hip_server *>(s)->mu);
   int64_t n = 0;
   switch (what) {
   case 0:

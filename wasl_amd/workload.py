@@ -119,6 +119,8 @@ def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True) 
         t0 = time.perf_counter()
         g = _engine(r, mesh)
         g.project(r.ic, r.params)
+        if r.limiting:
+            g.limit()   # the reference limits the projected state (euler_triangle.cpp:37-41)
         r.end_time = min(r.steps * g.stable_time_step(r.cfl), r.end_cap)
         r.dofs = g.E * g.Np * g.V
         try:
