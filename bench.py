@@ -83,42 +83,92 @@ def flops_per_element_stage(p: int) -> float:
     return vol + face + 3 * Np * V
 
 
-def cpu_baseline(mesh, order, cfl, budget_s: float):
-    """The reference's own CPU path (oracle/_ref, compiled from the reference
-    sources) or, where that was not built, the project's C++ restatement,
-    timed single-threaded on a bounded number of steps of the same case."""
+def _cpu_kind() -> str:
+    """The reference build the CPU baseline times: its -O3 x86-64-v3 build
+    where the host has AVX2 and FMA, else the parity build (-O2), else the
+    project's restatement."""
     from oracle import oracle as O
 
-    kind = "reference" if O.available("reference") else "port"
-    c = O.tri_euler(mesh.node_tags, mesh.node_xy, mesh.triangles, order, kind=kind)
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            flags = fh.read()
+    except OSError:
+        pass
+    if O.available("reference_fast") and " avx2" in flags and " fma" in flags:
+        return "reference_fast"
+    return "reference" if O.available("reference") else "port"
+
+
+def _cpu_worker(args):
+    """One independent vortex case on one core: (DOF-updates, seconds, steps)."""
+    node_tags, node_xy, triangles, order, cfl, budget_s, kind = args
+    from oracle import oracle as O
+
+    c = O.tri_euler(node_tags, node_xy, triangles, order, kind=kind)
     U = c.project(O.IC_VORTEX, (5.0, 5.0, 5.0))
     steps = 0
     t0 = time.perf_counter()
     while True:
-        dt = c.stable_dt(U, cfl)
-        U = c.step(U, dt)
+        U = c.step(U, c.stable_dt(U, cfl))
         steps += 1
         el = time.perf_counter() - t0
         if el > budget_s or steps >= 200:
             break
-    dofs = c.E * c.Np * c.V
-    return {
-        "value": dofs * steps / el,
+    return c.E * c.Np * c.V * steps, el, steps
+
+
+def cpu_baseline(mesh, order, cfl, budget_s: float, all_cores_s: float = 10.0):
+    """The reference's own CPU path (oracle/_ref, compiled from the reference
+    sources; its -O3 AVX2/FMA build where the host allows) timed on a bounded
+    number of steps of the same case: single-threaded, as the reference runs,
+    and, for the aggregate, one independent case per core (BASELINE's CPU
+    plan, item 4)."""
+    import multiprocessing as mp
+
+    kind = _cpu_kind()
+    build = {"reference_fast": "reference sources, g++ -O3 -march=x86-64-v3 (AVX2, FMA)",
+             "reference": "reference sources, g++ -O2 -ffp-contract=off (parity build)",
+             "port": "project restatement, g++ -O2"}[kind]
+    args = (mesh.node_tags, mesh.node_xy, mesh.triangles, order, cfl, budget_s, kind)
+    updates, el, steps = _cpu_worker(args)
+    out = {
+        "value": updates / el,
         "unit": "DOF-updates/s",
         "cores": 1,
-        "kind": kind,
-        "sample": f"{steps} SSP-RK3 steps of the same P2 32768-triangle vortex case from t=0 "
-                  f"({el:.1f} s, single thread, {cpu_model()}, nproc {os.cpu_count()})",
+        "kind": "port" if kind == "port" else "reference",
+        "build": build,
+        "sample": f"{steps} SSP-RK3 steps of the same P{order} {mesh.triangles.shape[0]}-triangle vortex "
+                  f"case from t=0 ({el:.1f} s, single thread, {cpu_model()}, nproc "
+                  f"{os.cpu_count()})",
     }
+    if all_cores_s > 0:
+        n = max(1, min(os.cpu_count() or 1, 16))
+        ctx = mp.get_context("spawn")
+        t0 = time.perf_counter()
+        with ctx.Pool(n) as pool:
+            res = pool.map(_cpu_worker, [args[:5] + (all_cores_s, kind)] * n)
+        wall = time.perf_counter() - t0
+        out["all_cores"] = {
+            "value": sum(u / e for u, e, _ in res),
+            "unit": "DOF-updates/s",
+            "cores": n,
+            "sample": f"{n} independent copies of the case, one per process, each timed over "
+                      f"up to {all_cores_s:.0f} s or 200 steps; value = sum of their rates "
+                      f"({wall:.1f} s wall including setup)",
+        }
+    return out
 
 
-def precision_error(mesh, p: int, precision: int, steps: int) -> dict:
+def precision_error(mesh, p: int, precision: int, steps: int, end_time: float = 0.0) -> dict:
     """The reduced-precision stage against the FP64 engine (the plain path,
-    parity-tested against the CPU oracle) on the bench workload: both start
-    from the same projected vortex and take `steps` steps of the same fixed
-    dt (the FP64 CFL step of the initial state). Density is compared as cell
-    averages (L1 weighted by element area) and as all modal coefficients
-    (L-inf), relative to the initial density's maximum cell average."""
+    parity-tested against the CPU oracle) on the bench workload, both from the
+    same projected vortex: `steps` steps of the same fixed dt (the FP64 CFL
+    step of the initial state), or, with end_time > 0, the device loop of
+    each engine to end_time, each choosing its own CFL dt and landing on
+    end_time exactly. Density is compared as cell averages (L1 weighted by
+    element area) and as all modal coefficients (L-inf), relative to the
+    initial density's maximum cell average."""
     import wasl_amd
 
     runs = {}
@@ -128,22 +178,34 @@ def precision_error(mesh, p: int, precision: int, steps: int) -> dict:
         if name == "fp64":
             dt = g.stable_time_step(0.15)
             rho0 = g.state.reshape(4, -1, g.E)[0, 0] / np.sqrt(2.0)
-        for _ in range(steps):
-            g.step(dt)
-        runs[name] = g.state.reshape(4, -1, g.E)[0]
+        t0 = time.perf_counter()
+        if end_time > 0:
+            t, n, _ = g.advance(0.15, end_time)
+        else:
+            for _ in range(steps):
+                g.step(dt)
+            t, n = steps * dt, steps
+        runs[name] = (g.state.reshape(4, -1, g.E)[0], t, n, time.perf_counter() - t0)
         E = g.E
         del g
-    d = runs["reduced"] - runs["fp64"]
+    d = runs["reduced"][0] - runs["fp64"][0]
     avg = d[0] / np.sqrt(2.0)
     scale = float(np.max(np.abs(rho0)))
     detj = mesh.prepared()["element_metrics"][:E, 4]   # proportional to the element's area
-    return {
-        "steps": steps, "dt": dt,
+    out = {
+        "steps": {k: v[2] for k, v in runs.items()},
+        "time": {k: v[1] for k, v in runs.items()},
+        "wall_s": {k: v[3] for k, v in runs.items()},
         "density_l1": float(np.sum(np.abs(avg) * detj) / np.sum(detj)),
         "density_linf": float(np.max(np.abs(d))),
         "density_linf_relative": float(np.max(np.abs(d))) / scale,
-        "reference": "FP64 engine, same initial state and fixed dt",
     }
+    if end_time > 0:
+        out["reference"] = (f"FP64 engine, same initial state, each engine's own device dt to "
+                            f"T = {end_time}")
+    else:
+        out.update(dt=dt, reference="FP64 engine, same initial state and fixed dt")
+    return out
 
 
 def rarefaction_run(cells: int, end_time: float) -> dict:
@@ -164,6 +226,7 @@ def rarefaction_run(cells: int, end_time: float) -> dict:
                         limiting=True, density_floor=1e-13, pressure_floor=1e-13)
     g.project(wasl_amd.IC_RAREFACTION, (1.0, 0.4, 2.0))
     g.limit()   # the reference limits the projected initial state
+    g.capture_graphs()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     out = {"workload": f"radial double rarefaction, P2, 2x{cells}x{cells}={g.E} jittered "
@@ -213,6 +276,7 @@ def advect2d_run(args) -> int:
                         wavespeed=vel[0], velocity_y=vel[1])
     g.project(wasl_amd.IC_SCALAR_WAVE, wave)
     E, Np = g.E, g.Np
+    g.capture_graphs()
     t, s, _ = g.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -221,12 +285,8 @@ def advect2d_run(args) -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = E * Np * args.steps / wall
-    g.set_timing(True)
-    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
-                        stop_at_max_steps=True)
-    g.set_timing(False)
-    _, stage_total, launches = g.last_timing()
-    stage_ms = stage_total / max(launches, 1)
+    finite = bool(np.isfinite(g.state).all())
+    stage_ms = float(np.mean(g.time_stages(args.stage_reps, g.stable_time_step(cfl)))) * 1e-3
     bytes_stage = E * advect_bytes_per_element_stage(Np)
     achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
     log(f"advect2d P{p}: {E} triangles, {wall * 1e3 / args.steps:.4f} ms/step, "
@@ -243,7 +303,7 @@ def advect2d_run(args) -> int:
                         f"2x{args.cells}x{args.cells}={E} jittered periodic triangles on "
                         f"[0,10]^2, upwind, SSP-RK3, CFL {cfl}, FP64",
             "elements": E, "modes": Np, "dofs": E * Np,
-            "state_finite": bool(np.isfinite(g.state).all()),
+            "state_finite": finite,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -305,6 +365,7 @@ def sod_run(args) -> int:
     g.project(wasl_amd.IC_EULER_REGIONS, sod)
     g.limit()
     E, Np = g.E, g.Np
+    g.capture_graphs()
     t, s, _ = g.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -313,12 +374,8 @@ def sod_run(args) -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = E * Np * 4 * args.steps / wall
-    g.set_timing(True)
-    t, s, _ = g.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
-                        stop_at_max_steps=True)
-    g.set_timing(False)
-    _, stage_total, launches = g.last_timing()
-    stage_ms = stage_total / max(launches, 1)
+    finite = bool(np.isfinite(g.state).all())
+    stage_ms = float(np.mean(g.time_stages(args.stage_reps, g.stable_time_step(cfl)))) * 1e-3
     bytes_stage = E * euler1d_bytes_per_element_stage(Np)
     achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
     log(f"sod P{p}: {E} cells, {wall * 1e3 / args.steps:.4f} ms/step, {value:.4e} DOF-updates/s, "
@@ -333,7 +390,7 @@ def sod_run(args) -> int:
             "workload": f"1D Euler Sod shock tube, P{p} modal Legendre, {E} uniform cells on "
                         f"[0,1], outflow ends, HLLC, char-WENO + positivity, CFL {cfl}, FP64",
             "elements": E, "modes": Np, "variables": 4, "dofs": E * Np * 4,
-            "state_finite": bool(np.isfinite(g.state).all()),
+            "state_finite": finite,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -374,10 +431,13 @@ def sod_run(args) -> int:
 def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
                         setup_threads: int = 4, fp64_tiles=None,
                         fp64_max_order: int = 1) -> dict:
-    """The serving mix run plain (FP64, one case at a time) and served
-    (mixed precision, continuous batching): aggregate DOF-updates/s of each,
-    and time to solution under Poisson arrivals at 80 % of the served
-    capacity measured closed-loop."""
+    """The serving mix served (mixed precision, continuous batching) and run
+    plain (FP64, one case at a time): aggregate DOF-updates/s of each, and
+    time to solution of both under the same Poisson arrivals at 80 % of the
+    served capacity measured closed-loop. Every tile-autotuning key of the
+    mix is tuned before either leg is timed. Requests that the reference's
+    own physicality checks stop (on the plain path) are listed and left out
+    of both rates."""
     import numpy as np
 
     from wasl_amd import workload as WL
@@ -385,59 +445,67 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
 
     mix = WL.serving_mix(count, seed)
     meshes = WL.MeshCache()
+    t0 = time.perf_counter()
     for r in mix:
         meshes.get(r.mesh_key)
-    log(f"serving mix: {count} requests ({sum(r.kind != 'advection' for r in mix)} Euler)")
-    WL.run_plain(mix, meshes)
-    failed = [r.index for r in mix if r.plain_error]
-    for r in mix:
-        if r.plain_error:
-            log(f"  plain stop: request {r.index} {r.kind} P{r.order} {r.size} elements, "
-                f"{r.steps} steps: {r.plain_error}")
-    mix = [r for r in mix if not r.plain_error]
-    count = len(mix)
-    plain_total = sum(r.plain_seconds for r in mix)
-    updates = sum(r.dofs * r.plain_steps for r in mix)
-    plain_rate = updates / plain_total
-    log(f"plain, one case at a time: {plain_total:.2f} s, {plain_rate:.4e} DOF-updates/s"
-        f" ({len(failed)} requests stopped by the reference's physicality checks, left out)")
+    keys = WL.warm_autotune(mix, meshes)
+    log(f"serving mix: {count} requests ({sum(r.kind != 'advection' for r in mix)} Euler), "
+        f"{len(meshes.meshes)} meshes and {keys} tile keys prepared in "
+        f"{time.perf_counter() - t0:.1f} s")
+    kw = dict(steps_per_batch=steps_per_batch, setup_threads=setup_threads,
+              fp64_tiles=fp64_tiles, fp64_max_order=fp64_max_order)
 
-    done, makespan, stats = WL.run_served(mix, meshes, steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles,
-                                          fp64_max_order=fp64_max_order)
-    served_updates = sum(mix[k].dofs * v[2] for k, v in done.items())
-    served_rate = served_updates / makespan
+    done, makespan, stats = WL.run_served(mix, meshes, **kw)
     capacity = count / makespan
-    log(f"served closed-loop: {makespan:.2f} s, {served_rate:.4e} DOF-updates/s, "
-        f"x{served_rate / plain_rate:.2f} plain; {stats}")
-
+    log(f"served closed-loop: {makespan:.2f} s for {count} requests; {stats}")
     rng = np.random.default_rng(seed + 1)
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, count)).tolist()
-    done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals,
-                                          steps_per_batch=steps_per_batch,
-                                          setup_threads=setup_threads, fp64_tiles=fp64_tiles,
-                                          fp64_max_order=fp64_max_order)
-    tts = [done_o[k][1] - done_o[k][0] for k in range(count)]
-    plain_tts = WL.fifo_times(arrivals, [r.plain_seconds for r in mix])
+    done_o, makespan_o, _ = WL.run_served(mix, meshes, arrivals=arrivals, **kw)
+    log(f"served open-loop at {rate:.2f} req/s: {makespan_o:.2f} s")
 
+    t0 = time.perf_counter()
+    WL.run_plain(mix, meshes, arrivals=arrivals)
+    plain_wall = time.perf_counter() - t0
+    failed = [k for k, r in enumerate(mix) if r.plain_error]
+    for k in failed:
+        r = mix[k]
+        log(f"  plain stop: request {r.index} {r.kind} P{r.order} {r.size} elements, "
+            f"{r.steps} steps: {r.plain_error}")
+    ok = [k for k in range(count) if not mix[k].plain_error]
+    served_failed = [k for k in ok if done[k][3] is None]
+    for k in served_failed:
+        r = mix[k]
+        log(f"  served stop: request {r.index} {r.kind} P{r.order} {r.size} elements")
+    both = [k for k in ok if done[k][3] is not None]
+    plain_total = sum(mix[k].plain_seconds for k in both)
+    updates = sum(mix[k].dofs * mix[k].plain_steps for k in both)
+    plain_rate = updates / plain_total
+    served_updates = sum(mix[k].dofs * done[k][2] for k in both)
+    served_rate = served_updates / makespan
+    end_mismatch = sum(1 for k in both if done[k][4] != mix[k].end_time)
+    log(f"plain: {plain_total:.2f} s of service ({plain_wall:.2f} s open-loop wall), "
+        f"{plain_rate:.4e} DOF-updates/s; served {served_rate:.4e} DOF-updates/s, "
+        f"x{served_rate / plain_rate:.2f} plain ({len(failed)} requests stopped by the "
+        f"reference's physicality checks, left out)")
+
+    tts = [done_o[k][1] - done_o[k][0] for k in both]
+    plain_tts = [mix[k].plain_done - arrivals[k] for k in both]
     worst = {}
-    served_failed = sum(1 for v in done.values() if v[3] is None)
-    for k, r in enumerate(mix):
-        if done[k][3] is None:
-            log(f"  served stop: request {r.index} {r.kind} P{r.order} {r.size} elements")
-            continue
+    for k in both:
+        r = mix[k]
         if r.kind == "advection":
             continue
         Np = (r.order + 1) * (r.order + 2) // 2
-        E = r.size
-        a = done[k][3].reshape(4, Np, E)[0]
-        b = r.plain_state.reshape(4, Np, E)[0]
+        a = done[k][3].reshape(4, Np, r.size)[0]
+        b = r.plain_state.reshape(4, Np, r.size)[0]
         worst[r.kind] = max(worst.get(r.kind, 0.0), float(np.max(np.abs(a - b))))
     out = {
         "requests": count,
+        "requests_measured": len(both),
         "requests_failed_plain": len(failed),
-        "requests_failed_served": served_failed,
+        "requests_failed_served_only": len(served_failed),
+        "served_end_time_mismatches": end_mismatch,
         "mix": "70% 2D Euler (vortex | radial rarefaction, limited | smooth Fourier), "
                "30% 1D advection; tris log-U[512,131072], cells log-U[256,65536], P U{1,2,3}, "
                "200-2000 steps",
@@ -457,16 +525,18 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         "tts_p99_s": WL.percentile(tts, 99),
         "plain_tts_p50_s": WL.percentile(plain_tts, 50),
         "plain_tts_p99_s": WL.percentile(plain_tts, 99),
+        "plain_tts_note": "measured: the plain path serves the same arrivals one case at a time "
+                          "in arrival order",
         "open_loop_makespan_s": makespan_o,
+        "plain_open_loop_wall_s": plain_wall,
         "closed_loop_server": stats,
         "density_linf_served_vs_fp64": worst,
         "density_linf_note": "max over requests of each kind of |rho coefficients, served "
-                             "(mixed) - plain (FP64)|; limited rarefactions can flip a "
-                             "troubled-cell decision between the two",
+                             "(mixed) - plain (FP64)|; limited cases run the FP64 stage",
     }
-    log(f"served open-loop at {rate:.2f} req/s: TTS p50 {out['tts_p50_s']:.3f} s "
-        f"p99 {out['tts_p99_s']:.3f} s (plain p50 {out['plain_tts_p50_s']:.3f} s "
-        f"p99 {out['plain_tts_p99_s']:.3f} s); density L-inf vs FP64 {worst}")
+    log(f"TTS p50 {out['tts_p50_s']:.3f} s p99 {out['tts_p99_s']:.3f} s (plain p50 "
+        f"{out['plain_tts_p50_s']:.3f} s p99 {out['plain_tts_p99_s']:.3f} s); density L-inf "
+        f"vs FP64 {worst}")
     return out
 
 
@@ -485,7 +555,7 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=["fp64", "mixed", "bf16"], default="fp64",
                     help="fp64: the plain drop-in; mixed: FP32 matrix-core contractions; "
                          "bf16: the BF16 ablation")
-    ap.add_argument("--serving-requests", type=int, default=64,
+    ap.add_argument("--serving-requests", type=int, default=1024,
                     help="requests of the serving mix (0: skip the serving measurement)")
     ap.add_argument("--serving-seed", type=int, default=0x5EED)
     ap.add_argument("--workload", choices=["vortex", "rarefaction", "advect2d", "sod"],
@@ -507,7 +577,14 @@ def main(argv=None) -> int:
                     help="tiles of the served FP64 2D Euler classes, order:kernel:lanes,... "
                          "(kernel 1 point split, 2 variable split); default: each case's "
                          "autotuned tile")
+    ap.add_argument("--stage-reps", type=int, default=100,
+                    help="back-to-back launches of each stage kernel for its duration")
+    ap.add_argument("--error-end-time", type=float, default=0.0,
+                    help="with --precision mixed/bf16: compare with the FP64 engine at this end "
+                         "time (each engine choosing its own dt) instead of --error-steps steps")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-all-cores-seconds", type=float, default=10.0,
+                    help="per-process budget of the all-cores CPU figure (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a plain copy kernel of known "
@@ -565,6 +642,9 @@ def main(argv=None) -> int:
         f"{eng.device_bytes / 2**20:.1f} MiB on device")
 
     cfl = 0.15
+    # setup: every step graph of the device loop is captured now, so none
+    # is captured inside the timed region whatever its length
+    eng.capture_graphs()
     t, s, _ = eng.advance(cfl, 1e9, max_steps=args.warmup, stop_at_max_steps=True)
     torch.cuda.synchronize()
     log(f"warmup {s} steps done (t = {t:.4f})")
@@ -579,22 +659,23 @@ def main(argv=None) -> int:
     value = E * Np * V * args.steps / wall
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4e} DOF-updates/s")
 
-    # Stage-kernel duration: HIP events around every stage kernel on the
-    # engine's stream over a second pass of the same length.
-    eng.set_timing(True)
-    t, s, _ = eng.advance(cfl, 1e9, max_steps=s + args.steps, time=t, step=s,
-                          stop_at_max_steps=True)
-    eng.set_timing(False)
-    _, stage_ms_total, launches = eng.last_timing()
-    stage_ms = stage_ms_total / max(launches, 1)
+    U = eng.state
+    finite = bool(np.isfinite(U).all())
+    # Stage-kernel duration: each stage launched back to back from a captured
+    # graph, HIP events on the engine's stream around the replay (the state
+    # is not meaningful afterwards, so it was read above).
+    stage_each = eng.time_stages(args.stage_reps, eng.stable_time_step(cfl))
+    stage_ms = float(np.mean(stage_each)) * 1e-3
+    assert 3 * stage_ms <= 1.02 * ms_per_step, (
+        f"stage kernel {stage_ms * 1e3:.2f} us x 3 exceeds the step time {ms_per_step * 1e3:.2f} us")
+    launches = 3 * args.stage_reps
     bytes_stage = E * algorithmic_bytes_per_element_step(Np, V) / 3.0
     achieved = bytes_stage / (stage_ms * 1e-3) / 1e9
     flops_stage = E * flops_per_element_stage(p)
-    log(f"stage kernel: {stage_ms * 1e3:.2f} us average over {launches} launches, "
-        f"{achieved:.1f} GB/s algorithmic, {flops_stage / (stage_ms * 1e-3) / 1e12:.2f} TFLOP/s")
+    log(f"stage kernel: {stage_ms * 1e3:.2f} us average over {launches} back-to-back launches "
+        f"(stages {', '.join(f'{x:.2f}' for x in stage_each)} us), {achieved:.1f} GB/s "
+        f"algorithmic, {flops_stage / (stage_ms * 1e-3) / 1e12:.2f} TFLOP/s")
 
-    U = eng.state
-    finite = bool(np.isfinite(U).all())
     kernel = ("whip::euler_stage_mx" if args.precision != "fp64" else
               ["whip::euler_stage", "whip::euler_stage_vs"][eng.stage_kernel - 1])
     tile = (f"<{p}>" if args.precision != "fp64" else
@@ -638,6 +719,9 @@ def main(argv=None) -> int:
             "traffic_frac_of_algorithmic": None if traffic is None else traffic / bytes_stage,
             "kernel": kernel + tile,
             "stage_kernel_us": stage_ms * 1e3,
+            "stage_kernel_us_by_stage": [float(x) for x in stage_each],
+            "stage_timing": f"{args.stage_reps} back-to-back launches of each stage replayed "
+                            f"from a captured graph, HIP events on the engine's stream",
             "algorithmic_bytes_per_launch": bytes_stage,
             "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,
             "fp64_frac": flops_stage / (stage_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
@@ -645,8 +729,9 @@ def main(argv=None) -> int:
         "cpu_baseline": None,
     }
     if args.precision != "fp64":
-        result["precision_error"] = precision_error(mesh, p, precision, args.error_steps)
-        log(f"{args.precision} vs FP64 after {args.error_steps} steps: density L1 "
+        result["precision_error"] = precision_error(mesh, p, precision, args.error_steps,
+                                                    args.error_end_time)
+        log(f"{args.precision} vs FP64 ({result['precision_error']['steps']} steps): density L1 "
             f"{result['precision_error']['density_l1']:.3e}, L-inf "
             f"{result['precision_error']['density_linf']:.3e}")
     if args.serving_requests > 0:
@@ -658,7 +743,8 @@ def main(argv=None) -> int:
                                                 tiles or None, args.serving_fp64_max_order)
     if not args.no_cpu:
         log(f"cpu baseline: timing up to {args.cpu_seconds:.0f} s of the CPU path")
-        result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(mesh_in, p, cfl, args.cpu_seconds,
+                                              args.cpu_all_cores_seconds)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.4e} DOF-updates/s")
     print(json.dumps(result), flush=True)
     return 0

@@ -272,6 +272,19 @@ int wasl_hip_last_timing(const wasl_hip_engine *engine, double *wall_ms, double 
  * wasl_hip_last_timing can report the stage kernel's average duration. */
 int wasl_hip_set_timing(wasl_hip_engine *engine, int on);
 
+/* Not part of the reference interface. wasl_hip_advance replays its steps
+ * from captured HIP graphs of 1, 2, 4, ..., 64 steps at either step parity,
+ * captured when first needed; this captures them all now (setup), so that no
+ * capture falls inside a caller's timed or latency-critical loop. */
+int wasl_hip_capture_graphs(wasl_hip_engine *engine);
+
+/* Not part of the reference interface: the stage kernel's average duration
+ * in microseconds per stage (stage_us[3]), each stage launched reps times
+ * back to back from a captured graph with fixed dt and timed with HIP events
+ * on the engine's stream (benchmarks). The resident state is not meaningful
+ * afterwards. */
+int wasl_hip_time_stages(wasl_hip_engine *engine, int reps, double dt, double *stage_us);
+
 #ifdef __cplusplus
 }
 #endif

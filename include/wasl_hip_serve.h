@@ -72,8 +72,9 @@ void wasl_hip_server_destroy(wasl_hip_server *server);
 /* Queues one case. The mesh is copied (it may be destroyed afterwards); the
  * numerics are those of wasl_hip_engine_create (the server's precision applies
  * to 2D Euler); ic/params as wasl_hip_project. The case runs to end_time, or,
- * when end_steps > 0, to end_steps times the CFL step of its projected initial
- * state; max_steps bounds the step count. */
+ * when end_steps > 0, to end_steps times the CFL step of its projected (and,
+ * with limiting, limited) initial state, at most to end_time when end_time is
+ * positive as well; max_steps bounds the step count. */
 int wasl_hip_server_submit(wasl_hip_server *server, const wasl_hip_mesh *mesh,
                            const wasl_hip_numerics *numerics, int ic, const double *params,
                            int64_t param_count, double cfl, double end_time, uint64_t end_steps,

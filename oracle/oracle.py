@@ -6,7 +6,9 @@ measured on the GPU. Two backends implement the same API:
 
   "port"       oracle/liboracle.so, the project's C++ restatement;
   "reference"  oracle/_ref/libwaslref.so, the reference's own sources
-               compiled by oracle/Makefile (present only where it was built).
+               compiled by oracle/Makefile (present only where it was built);
+  "reference_fast"  the same sources built for speed, only for bench.py's
+               timed CPU baseline.
 """
 
 from __future__ import annotations
@@ -20,6 +22,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PATHS = {
     "port": os.path.join(HERE, "liboracle.so"),
     "reference": os.path.join(HERE, "_ref", "libwaslref.so"),
+    # the same reference sources built -O3 -march=x86-64-v3 (AVX2 and FMA,
+    # contraction on): the timed CPU baseline, never a parity checker
+    "reference_fast": os.path.join(HERE, "_ref", "libwaslref_fast.so"),
 }
 
 PERIODIC, INFLOW, OUTFLOW = 0, 1, 2

@@ -148,13 +148,18 @@ struct wasl_hip_engine {
   int64_t last_stage_launches = 0;
   bool timing = false;
 
-  // captured batch of steps for wasl_hip_advance
-  hipGraphExec_t graph_exec = nullptr;
+  // Captured step graphs for wasl_hip_advance: graphs[k][parity] holds 2^k
+  // steps starting at that step parity, so a batch of any length replays as
+  // at most kGraphLevels graphs.
+  static constexpr int kGraphLevels = 7;   // 1 .. 64 steps
+  hipGraphExec_t graphs[kGraphLevels][2] = {};
   bool use_graphs = true;
 
   ~wasl_hip_engine() {
-    if (graph_exec)
-      (void)hipGraphExecDestroy(graph_exec);
+    for (auto &level : graphs)
+      for (hipGraphExec_t x : level)
+        if (x)
+          (void)hipGraphExecDestroy(x);
     for (auto e : ev)
       (void)hipEventDestroy(e);
     if (ctl_host)
@@ -679,38 +684,44 @@ void launch_sensor(const wasl_hip_engine &g, const double *U, uint8_t *flags,
 
 bool limiting(const wasl_hip_engine &g) { return g.euler() && g.num.limiting && g.p > 0; }
 
-// Shu-Osher stages (time/ssp_rk3.hpp:45-60): U1 = U + dt R(U);
-// U2 = 3/4 U + 1/4 (U1 + dt R(U1)); U = 1/3 U + 2/3 (U2 + dt R(U2)); the
-// limiter after each.
-void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, bool timed,
-                 int &timed_index) {
+// Arguments of Shu-Osher stage s (time/ssp_rk3.hpp:45-60): U1 = U + dt R(U);
+// U2 = 3/4 U + 1/4 (U1 + dt R(U1)); U = 1/3 U + 2/3 (U2 + dt R(U2)).
+whip::StageArgs stage_args(const wasl_hip_engine &g, int s, double dt, whip::RunCtl *ctl,
+                           int parity, bool sensor) {
   double *U = g.u.as<double>(), *A = g.w1.as<double>(), *B = g.w2.as<double>();
   const double *in[3] = {U, A, B};
   double *out[3] = {A, B, U};
   const double ca[3] = {0.0, 0.75, 1.0 / 3.0};
   const double cb[3] = {1.0, 0.25, 2.0 / 3.0};
+  whip::StageArgs a{};
+  a.uin = in[s];
+  a.un = U;
+  a.out = out[s];
+  a.ca = ca[s];
+  a.cb = cb[s];
+  a.dt = dt;
+  a.ctl = ctl;
+  a.parity = parity;
+  a.stage = s;
+  a.residual_only = 0;
+  a.want_lambda = (g.euler() && ctl && s == 2) ? 1 : 0;
+  a.sensor = sensor ? 1 : 0;
+  a.threshold = g.num.sensor_threshold;
+  a.highest_begin = g.stencil.highest_begin;
+  a.flags = nullptr;
+  a.list = g.list.as<int>();
+  a.counters = g.counters.as<int>();
+  a.tin = g.traces_of(in[s]);
+  a.tout = g.traces_of(out[s]);
+  return a;
+}
+
+// One SSP-RK3 step: the three fused stages, the limiter after each.
+void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, bool timed,
+                 int &timed_index) {
   const bool lim = limiting(g);
   for (int s = 0; s < 3; ++s) {
-    whip::StageArgs a{};
-    a.uin = in[s];
-    a.un = U;
-    a.out = out[s];
-    a.ca = ca[s];
-    a.cb = cb[s];
-    a.dt = dt;
-    a.ctl = ctl;
-    a.parity = parity;
-    a.stage = s;
-    a.residual_only = 0;
-    a.want_lambda = (g.euler() && ctl && s == 2) ? 1 : 0;
-    a.sensor = lim ? 1 : 0;
-    a.threshold = g.num.sensor_threshold;
-    a.highest_begin = g.stencil.highest_begin;
-    a.flags = nullptr;
-    a.list = g.list.as<int>();
-    a.counters = g.counters.as<int>();
-    a.tin = g.traces_of(in[s]);
-    a.tout = g.traces_of(out[s]);
+    const whip::StageArgs a = stage_args(g, s, dt, ctl, parity, lim);
     if (timed)
       HIPCHECK(hipEventRecord(g.ev[2 * timed_index], g.stream));
     launch_stage(g, a);
@@ -719,7 +730,7 @@ void launch_step(wasl_hip_engine &g, double dt, whip::RunCtl *ctl, int parity, b
       ++timed_index;
     }
     if (lim)
-      launch_limit(g, out[s], s, ctl, parity);
+      launch_limit(g, a.out, s, ctl, parity);
   }
 }
 
@@ -1228,6 +1239,26 @@ void limit_resident(wasl_hip_engine &g, uint8_t *flags) {
     HIPCHECK(hipMemcpyAsync(flags, g.flags.p, g.E, hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(hipStreamSynchronize(g.stream));
   raise_device_error(g);
+}
+
+// The captured graph of 2^k device-driven steps starting at step parity
+// `parity`, captured on first use. Steps change parity one by one, so a
+// graph of an even number of steps also ends at `parity`.
+hipGraphExec_t step_graph(wasl_hip_engine &g, int k, int parity, whip::RunCtl *dctl) {
+  hipGraphExec_t &x = g.graphs[k][parity];
+  if (!x) {
+    hipGraph_t graph;
+    HIPCHECK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    int par = parity, unused = 0;
+    for (int i = 0; i < (1 << k); ++i) {
+      launch_step(g, 0.0, dctl, par, false, unused);
+      par ^= 1;
+    }
+    HIPCHECK(hipStreamEndCapture(g.stream, &graph));
+    HIPCHECK(hipGraphInstantiate(&x, graph, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(graph));
+  }
+  return x;
 }
 
 }  // namespace
@@ -1836,22 +1867,13 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
         batch = std::min<uint64_t>(batch, to_next);
       }
       int idx = 0;
-      if (!timed && batch == kBatch && parity == 0 && g->use_graphs) {
-        // A full batch of an even number of steps returns to parity 0, so one
-        // captured graph serves every full batch.
-        if (!g->graph_exec) {
-          hipGraph_t graph;
-          HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-          int par = 0, unused = 0;
-          for (int i = 0; i < kBatch; ++i) {
-            launch_step(*g, 0.0, dctl, par, false, unused);
-            par ^= 1;
+      if (!timed && g->use_graphs) {
+        // the batch's binary digits, largest first: one graph each
+        for (int k = wasl_hip_engine::kGraphLevels - 1; k >= 0; --k)
+          if (batch & (1ull << k)) {
+            HIPCHECK(hipGraphLaunch(step_graph(*g, k, parity, dctl), g->stream));
+            parity ^= k == 0 ? 1 : 0;
           }
-          HIPCHECK(hipStreamEndCapture(g->stream, &graph));
-          HIPCHECK(hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0));
-          HIPCHECK(hipGraphDestroy(graph));
-        }
-        HIPCHECK(hipGraphLaunch(g->graph_exec, g->stream));
       } else {
         for (uint64_t i = 0; i < batch; ++i) {
           launch_step(*g, 0.0, dctl, parity, timed, idx);
@@ -1939,6 +1961,62 @@ int wasl_hip_last_timing(const wasl_hip_engine *g, double *wall_ms, double *stag
       *stage_ms = g->last_stage_ms;
     if (stage_launches)
       *stage_launches = g->last_stage_launches;
+  });
+}
+
+// Not part of the reference interface: the stage kernel's duration. Each
+// of the three stages (fixed dt, no step control, no limiter) is launched
+// `reps` times back to back from one captured graph and timed with HIP
+// events on the engine's stream; stage_us[s] receives the average per
+// launch. Stages 1 and 2 repeat exactly; stage 3 overwrites U^n, which it
+// also reads, so the resident state is not meaningful afterwards.
+int wasl_hip_time_stages(wasl_hip_engine *g, int reps, double dt, double *stage_us) {
+  return guarded([&] {
+    if (!g || !stage_us || reps < 1 || reps > 4096)
+      throw std::runtime_error("engine: time_stages needs an engine, 1..4096 repetitions");
+    hipEvent_t t0, t1;
+    HIPCHECK(hipEventCreate(&t0));
+    HIPCHECK(hipEventCreate(&t1));
+    for (int s = 0; s < 3; ++s) {
+      const whip::StageArgs a = stage_args(*g, s, dt, nullptr, 0, false);
+      if (g->mixed())
+        launch_traces(*g, a.uin, const_cast<float *>(a.tin), g->mxtable.as<float>(), g->stream);
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      HIPCHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < reps; ++i)
+        launch_stage(*g, a);
+      HIPCHECK(hipStreamEndCapture(g->stream, &graph));
+      HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      HIPCHECK(hipGraphDestroy(graph));
+      HIPCHECK(hipGraphLaunch(exec, g->stream));   // warm
+      HIPCHECK(hipEventRecord(t0, g->stream));
+      HIPCHECK(hipGraphLaunch(exec, g->stream));
+      HIPCHECK(hipEventRecord(t1, g->stream));
+      HIPCHECK(hipEventSynchronize(t1));
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+      (void)hipGraphExecDestroy(exec);
+      stage_us[s] = 1e3 * static_cast<double>(ms) / reps;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+  });
+}
+
+// Not part of the reference interface: captures every step graph of
+// wasl_hip_advance ahead of time (setup), so that no capture falls inside a
+// caller's timed or latency-critical stepping loop.
+int wasl_hip_capture_graphs(wasl_hip_engine *g) {
+  return guarded([&] {
+    if (!g)
+      throw std::runtime_error("engine: null argument");
+    if (!g->use_graphs)
+      return;
+    for (int k = 0; k < wasl_hip_engine::kGraphLevels; ++k)
+      for (int parity = 0; parity < 2; ++parity)
+        step_graph(*g, k, parity, g->ctl.as<whip::RunCtl>());
+    HIPCHECK(hipStreamSynchronize(g->stream));
   });
 }
 

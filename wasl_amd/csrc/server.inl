@@ -185,7 +185,8 @@ void prepare(Request &r) {
                                 : 0.0;
       if (!(stable > 0.0))
         throw SimulationError("simulation: timestep cannot advance time");
-      r.end_time = static_cast<double>(r.end_steps) * stable;
+      const double by_steps = static_cast<double>(r.end_steps) * stable;
+      r.end_time = r.end_time > 0.0 ? std::min(by_steps, r.end_time) : by_steps;
     }
     HIPCHECK(hipStreamSynchronize(g->stream));
     r.eng = std::move(g);

@@ -105,6 +105,8 @@ def library() -> C.CDLL:
         "wasl_hip_cell_averages": [_P, _pd],
         "wasl_hip_last_timing": [_P, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
         "wasl_hip_set_timing": [_P, C.c_int],
+        "wasl_hip_capture_graphs": [_P],
+        "wasl_hip_time_stages": [_P, C.c_int, _d, _pd],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -397,6 +399,17 @@ class Engine:
         out = np.zeros(self.E * self.V)
         _check(library().wasl_hip_cell_averages(self.h, out))
         return out.reshape(self.E, -1)
+
+    def capture_graphs(self) -> None:
+        """Capture every step graph of advance() now (setup)."""
+        _check(library().wasl_hip_capture_graphs(self.h))
+
+    def time_stages(self, reps: int, dt: float) -> np.ndarray:
+        """Average duration (us) of each stage kernel launched reps times back
+        to back; leaves the resident state not meaningful."""
+        out = np.zeros(3)
+        _check(library().wasl_hip_time_stages(self.h, reps, dt, out))
+        return out
 
     def set_timing(self, on: bool) -> None:
         _check(library().wasl_hip_set_timing(self.h, int(on)))

@@ -11,7 +11,8 @@ at t = 0.3 at the latest (the near-vacuum time of the rarefaction case); vortex
 and smooth fields on periodic [0,10]^2 at CFL 0.15; advection of the pulse on
 periodic [0,1] at CFL 0.1.
 
-plain:  one case at a time through the FP64 engine (create, project, advance)
+plain:  one case at a time through the FP64 engine (create, project, limit
+        when limiting, advance), in arrival order as requests arrive
 served: all cases through a Server (mixed precision, continuous batching)
 """
 
@@ -49,6 +50,7 @@ class Request:
     plain_steps: int = 0
     plain_state: np.ndarray | None = field(default=None, repr=False)
     plain_error: str = ""
+    plain_done: float = 0.0
     end_cap: float = float("inf")
 
 
@@ -106,14 +108,27 @@ def _engine(r: Request, mesh, precision=W.FP64):
                     wavespeed=1.0)
 
 
-def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True) -> float:
-    """One case at a time, FP64: engine setup, projection and the stepping
-    loop are the service time. Sets each request's end time (its steps times
-    the initial CFL step) and DOF count; returns the total seconds."""
+def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True,
+              arrivals=None) -> float:
+    """One case at a time, FP64: engine setup, projection (and the initial
+    limit of limited cases) and the stepping loop are the service time. With
+    arrivals (offsets in seconds from the start) the requests are served in
+    arrival order as they arrive, a single FIFO server that idles until the
+    next arrival when it has caught up; each request's completion offset goes
+    to plain_done. Sets each request's end time (its steps times the initial
+    CFL step) and DOF count; returns the total service seconds."""
     import torch
 
+    order = list(range(len(reqs))) if arrivals is None else \
+        sorted(range(len(reqs)), key=lambda k: arrivals[k])
     total = 0.0
-    for r in reqs:
+    start = time.perf_counter()
+    for k in order:
+        r = reqs[k]
+        if arrivals is not None:
+            wait = arrivals[k] - (time.perf_counter() - start)
+            if wait > 0:
+                time.sleep(wait)
         mesh = meshes.get(r.mesh_key)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -133,10 +148,28 @@ def run_plain(reqs: list[Request], meshes: MeshCache, keep_states: bool = True) 
             # euler_positivity.hpp:74-75); the request is left out of the rates
             r.plain_error = str(exc)
         torch.cuda.synchronize()
-        r.plain_seconds = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        r.plain_seconds = t1 - t0
+        r.plain_done = t1 - start
         total += r.plain_seconds
         del g
     return total
+
+
+def warm_autotune(reqs: list[Request], meshes: MeshCache) -> int:
+    """Creates one FP64 engine per tile-autotuning key of the mix (2D Euler:
+    order, limiting and the engine's mesh-size class), so neither timed leg
+    pays the process-wide tile search. Returns the number of keys."""
+    keys = {}
+    for r in reqs:
+        if r.equation != W.EULER:
+            continue
+        size_class = 0 if r.size < 8192 else 1 if r.size < 131072 else 2
+        keys.setdefault((r.order, r.limiting, size_class), r)
+    for r in keys.values():
+        g = _engine(r, meshes.get(r.mesh_key))
+        del g
+    return len(keys)
 
 
 def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=W.MIXED,
@@ -163,8 +196,12 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
         while pending and (arrivals is None or arrivals[pending[0]] <= now):
             k = pending.pop(0)
             r = reqs[k]
+            # the server fixes the end time as the plain run does: the
+            # request's steps times its initial CFL step, capped at end_cap
             rid = s.submit(meshes.get(r.mesh_key), r.equation, r.order, r.ic, r.params, cfl=r.cfl,
-                           end_time=r.end_time, boundary=r.boundary, limiting=r.limiting,
+                           end_steps=r.steps,
+                           end_time=r.end_cap if math.isfinite(r.end_cap) else 0.0,
+                           boundary=r.boundary, limiting=r.limiting,
                            density_floor=r.floors, pressure_floor=r.floors, wavespeed=1.0)
             ids[rid] = (k, now if arrivals is not None else 0.0)
         if s.idle and not s.count(3):
@@ -177,26 +214,14 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
             try:
                 state, t, steps = s.result(rid)
             except W.WaslHipError:
-                state, steps = None, 0
-            done[k] = (arrived, finish, steps, state)
+                state, t, steps = None, float("nan"), 0
+            done[k] = (arrived, finish, steps, state, t)
     makespan = time.perf_counter() - t0
     stats = {"rounds": s.count(6), "graphs": s.count(5), "resident_changes": s.count(7),
              "rebuild_s": s.count(8) * 1e-6, "round_wait_s": s.count(9) * 1e-6,
              "prepare_s": s.count(10) * 1e-6}
     s.close()
     return done, makespan, stats
-
-
-def fifo_times(arrivals, service):
-    """Time to solution of a single FIFO server with the given arrival
-    offsets and service times."""
-    free = 0.0
-    out = []
-    for a, s in sorted(zip(arrivals, service)):
-        start = max(a, free)
-        free = start + s
-        out.append(free - a)
-    return out
 
 
 def percentile(xs, q):
