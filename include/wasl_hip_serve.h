@@ -102,7 +102,10 @@ int64_t wasl_hip_server_state_size(const wasl_hip_server *server, uint64_t id);
  * unclaimed, 4 device bytes reserved, 5 graphs captured, 6 rounds run,
  * 7 resident-set changes, 8 microseconds spent rebuilding launch tables and
  * graphs, 9 microseconds spent waiting for rounds to complete, 10
- * microseconds of case preparation (summed over the setup threads). */
+ * microseconds of case preparation (summed over the setup threads),
+ * microseconds spent 11 admitting prepared cases, 12 retiring finished ones,
+ * 13 waiting for a preparation with nothing resident, 14 in submit, 15
+ * releasing retired cases' device memory (part of 12). */
 int64_t wasl_hip_server_count(const wasl_hip_server *server, int what);
 
 #ifdef __cplusplus

@@ -71,24 +71,37 @@ const char *device_error_text(int code) {
   }
 }
 
-// Device allocation owned by the engine.
+// Device allocation owned by the engine. With a stream it is stream-ordered
+// (hipMallocAsync / hipFreeAsync from the device's memory pool): releasing it
+// neither synchronises the device nor returns memory to the driver, which
+// makes creating and destroying many engines (one per served case) cheap.
 struct Buffer {
   void *p = nullptr;
   size_t bytes = 0;
+  hipStream_t stream = nullptr;
   Buffer() = default;
   Buffer(const Buffer &) = delete;
   Buffer &operator=(const Buffer &) = delete;
-  ~Buffer() {
-    if (p)
-      (void)hipFree(p);
-  }
-  void alloc(size_t n, size_t &total) {
-    if (p)
-      (void)hipFree(p);
+  ~Buffer() { release(); }
+  void release() {
+    if (p) {
+      if (stream)
+        (void)hipFreeAsync(p, stream);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n, size_t &total, hipStream_t s = nullptr) {
+    release();
     bytes = n;
+    stream = s;
     if (n) {
-      HIPCHECK(hipMalloc(&p, n));
+      if (s)
+        HIPCHECK(hipMallocAsync(&p, n, s));
+      else
+        HIPCHECK(hipMalloc(&p, n));
       total += n;
     }
   }
@@ -97,9 +110,93 @@ struct Buffer {
 
 template <typename T>
 void upload(Buffer &b, const std::vector<T> &v, size_t &total, hipStream_t s) {
-  b.alloc(v.size() * sizeof(T), total);
+  b.alloc(v.size() * sizeof(T), total, s);
   if (!v.empty())
     HIPCHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Memory freed by stream-ordered releases stays in the device's pool for the
+// next engine instead of going back to the driver at each synchronisation.
+void keep_pool_memory(int device) {
+  static std::mutex mu;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (std::find(done.begin(), done.end(), device) != done.end())
+    return;
+  hipMemPool_t pool;
+  HIPCHECK(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t keep = ~0ull;
+  HIPCHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  done.push_back(device);
+}
+
+// Small pinned host slots for step-control read-backs. Pinning and unpinning
+// per engine costs about a millisecond each way, so slots come from 64 KiB
+// pinned chunks kept for the life of the process.
+constexpr size_t kPinnedSlot = 256;
+
+class PinnedSlots {
+public:
+  void *get() {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (free_.empty()) {
+      char *chunk = nullptr;
+      HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&chunk), kChunk));
+      for (size_t i = 0; i < kChunk / kPinnedSlot; ++i)
+        free_.push_back(chunk + i * kPinnedSlot);
+    }
+    void *slot = free_.back();
+    free_.pop_back();
+    return slot;
+  }
+  void put(void *slot) {
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.push_back(slot);
+  }
+
+private:
+  static constexpr size_t kChunk = size_t(64) << 10;
+  std::mutex mu_;
+  std::vector<void *> free_;
+};
+
+PinnedSlots &pinned_slots() {
+  static PinnedSlots *slots = new PinnedSlots;   // outlives every engine
+  return *slots;
+}
+
+// Engine streams, reused: creating and destroying a HIP stream per engine
+// costs about a millisecond. A returned stream may still hold its engine's
+// stream-ordered frees; the next engine's work simply queues behind them.
+class StreamPool {
+public:
+  hipStream_t get(int device) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (size_t i = 0; i < free_.size(); ++i)
+        if (free_[i].first == device) {
+          hipStream_t s = free_[i].second;
+          free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(i));
+          return s;
+        }
+    }
+    hipStream_t s;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+  }
+  void put(int device, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.emplace_back(device, s);
+  }
+
+private:
+  std::mutex mu_;
+  std::vector<std::pair<int, hipStream_t>> free_;
+};
+
+StreamPool &stream_pool() {
+  static StreamPool *pool = new StreamPool;   // outlives every engine
+  return *pool;
 }
 
 // Default tile shape of the Euler stage kernel when the caller names none.
@@ -163,11 +260,16 @@ struct wasl_hip_engine {
     for (auto e : ev)
       (void)hipEventDestroy(e);
     if (ctl_host)
-      (void)hipHostFree(ctl_host);
+      pinned_slots().put(ctl_host);
     if (result_host)
-      (void)hipHostFree(result_host);
+      pinned_slots().put(result_host);
+    // stream-ordered releases need the stream: before it goes
+    for (Buffer *b : {&u, &w1, &w2, &geo, &nbr, &meta, &fn, &bdonor, &bbasis, &table, &vtable,
+                      &inv_jac, &mxtable, &tr_u, &tr_w1, &tr_w2, &lnbr, &T, &nrm, &check,
+                      &flags, &list, &counters, &side, &err, &ctl, &mass_w, &partials, &result})
+      b->release();
     if (stream)
-      (void)hipStreamDestroy(stream);
+      stream_pool().put(device, stream);
   }
 
   size_t state_size() const { return static_cast<size_t>(V) * Np * E; }
@@ -979,11 +1081,13 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     throw std::runtime_error("engine: precision must be FP64, MIXED or BF16");
   g.mesh = mesh;
   HIPCHECK(hipGetDevice(&g.device));
-  HIPCHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  g.stream = stream_pool().get(g.device);
   if (const char *env = std::getenv("WASL_HIP_GRAPHS"))
     g.use_graphs = std::atoi(env) != 0;
-  HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.ctl_host), sizeof(whip::RunCtl)));
-  HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&g.result_host), sizeof(double)));
+  static_assert(sizeof(whip::RunCtl) <= kPinnedSlot, "step control fits a pinned slot");
+  g.ctl_host = static_cast<whip::RunCtl *>(pinned_slots().get());
+  g.result_host = static_cast<double *>(pinned_slots().get());
+  keep_pool_memory(g.device);
   size_t &tot = g.device_bytes;
 
   if (num.equation == WASL_HIP_EULER && mesh.dimension == 1) {
@@ -1029,12 +1133,12 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
       }
       upload(g.nrm, nrm, tot, g.stream);
       upload(g.check, g.stencil.check, tot, g.stream);
-      g.side.alloc(E * 4 * g.Np * sizeof(double), tot);
+      g.side.alloc(E * 4 * g.Np * sizeof(double), tot, g.stream);
     } else {
       g.stencil.highest_begin = g.p > 0 ? g.Np - 1 : g.Np;
     }
-    g.flags.alloc(E, tot);
-    g.list.alloc(E * sizeof(int), tot);
+    g.flags.alloc(E, tot, g.stream);
+    g.list.alloc(E * sizeof(int), tot, g.stream);
   } else if (num.equation == WASL_HIP_EULER) {
     if (mesh.dimension != 2)
       throw std::runtime_error("engine: Euler runs on line or triangle meshes");
@@ -1110,12 +1214,12 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
       }
       upload(g.nrm, nrm, tot, g.stream);
       upload(g.check, g.stencil.check, tot, g.stream);
-      g.side.alloc(E * 4 * g.Np * sizeof(double), tot);
+      g.side.alloc(E * 4 * g.Np * sizeof(double), tot, g.stream);
     } else {
       g.stencil.highest_begin = g.p > 0 ? whip::tri_modes(g.p - 1) : g.Np;
     }
-    g.flags.alloc(E, tot);
-    g.list.alloc(E * sizeof(int), tot);
+    g.flags.alloc(E, tot, g.stream);
+    g.list.alloc(E * sizeof(int), tot, g.stream);
   } else if (num.equation == WASL_HIP_SCALAR_ADVECTION && mesh.dimension == 2) {
     // src/case/scalar_advection_triangle.cpp:15-46: periodic box, constant
     // velocity (wavespeed, velocity_y), no limiter.
@@ -1197,23 +1301,23 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
     throw std::runtime_error("engine: unknown equation");
   }
   const size_t n = g.state_size() * sizeof(double);
-  g.u.alloc(n, tot);
-  g.w1.alloc(n, tot);
-  g.w2.alloc(n, tot);
+  g.u.alloc(n, tot, g.stream);
+  g.w1.alloc(n, tot, g.stream);
+  g.w2.alloc(n, tot, g.stream);
   if (g.mixed()) {
     const size_t tb = static_cast<size_t>(g.E) * 48 * sizeof(float);
-    g.tr_u.alloc(tb, tot);
-    g.tr_w1.alloc(tb, tot);
-    g.tr_w2.alloc(tb, tot);
+    g.tr_u.alloc(tb, tot, g.stream);
+    g.tr_w1.alloc(tb, tot, g.stream);
+    g.tr_w2.alloc(tb, tot, g.stream);
   }
   HIPCHECK(hipMemsetAsync(g.u.p, 0, n, g.stream));
-  g.counters.alloc(4 * sizeof(int), tot);
+  g.counters.alloc(4 * sizeof(int), tot, g.stream);
   HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
-  g.err.alloc(2 * sizeof(int), tot);
+  g.err.alloc(2 * sizeof(int), tot, g.stream);
   reset_device_error(g);
-  g.ctl.alloc(sizeof(whip::RunCtl), tot);
-  g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot);
-  g.result.alloc(sizeof(double), tot);
+  g.ctl.alloc(sizeof(whip::RunCtl), tot, g.stream);
+  g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot, g.stream);
+  g.result.alloc(sizeof(double), tot, g.stream);
   HIPCHECK(hipStreamSynchronize(g.stream));
   if (g.euler() && g.dim == 2 && num.precision == WASL_HIP_FP64 && num.stage_kernel == 0 &&
       num.lanes_per_element == 0)

@@ -114,7 +114,8 @@ struct wasl_hip_server {
   std::vector<uint64_t> finished;    // done and unclaimed
   bool dirty = true;
   int64_t graphs = 0, rounds = 0, changes = 0;
-  int64_t rebuild_us = 0, wait_us = 0;
+  int64_t rebuild_us = 0, wait_us = 0, admit_us = 0, retire_us = 0, idle_us = 0, submit_us = 0;
+  int64_t release_us = 0;
   std::atomic<int64_t> prepare_us{0};
 
   // launch classes of the resident set and the round's step-control gather
@@ -554,7 +555,11 @@ void retire(wasl_hip_server &s, Request &r, int status, const std::string &messa
                      hipMemcpyDeviceToHost));
   r.status = status;
   r.message = message;
+  const auto t_release = std::chrono::steady_clock::now();
   r.eng.reset();
+  s.release_us += std::chrono::duration_cast<std::chrono::microseconds>(
+                      std::chrono::steady_clock::now() - t_release)
+                      .count();
   s.reserved -= r.reserved;
   r.reserved = 0;
   r.phase = Phase::done;
@@ -633,6 +638,7 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
     if (!s || !mesh || !numerics || !id || (param_count > 0 && !params))
       throw std::runtime_error("server: null argument");
     std::lock_guard<std::mutex> api(s->api_mu);
+    const auto t_submit = std::chrono::steady_clock::now();
     if (!(cfl > 0.0) || (end_steps == 0 && !(end_time > 0.0)) || max_steps == 0)
       throw std::runtime_error("server: invalid run control");
     const bool euler = numerics->equation == WASL_HIP_EULER;
@@ -666,6 +672,9 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
     *id = r->id;
     s->queue.push_back(r->id);
     s->requests[r->id] = std::move(r);
+    s->submit_us += std::chrono::duration_cast<std::chrono::microseconds>(
+                        std::chrono::steady_clock::now() - t_submit)
+                        .count();
   });
 }
 
@@ -704,7 +713,11 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
               return true;
           return false;
         };
+        const auto t_idle = std::chrono::steady_clock::now();
         s->cv_ready.wait_for(lock, std::chrono::milliseconds(50), has_ready);
+        s->idle_us += std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - t_idle)
+                          .count();
       }
       // failed preparations finish here; ready ones are admitted
       for (auto &[id, r] : s->requests) {
@@ -718,6 +731,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
         }
       }
     }
+    const auto t_admit = std::chrono::steady_clock::now();
     for (uint64_t id : ready_now) {
       Request *r = s->requests.at(id).get();
       // replace the estimate by what the engine holds
@@ -727,6 +741,9 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
       s->resident.push_back(id);
       s->dirty = true;
     }
+    s->admit_us += std::chrono::duration_cast<std::chrono::microseconds>(
+                       std::chrono::steady_clock::now() - t_admit)
+                       .count();
     if (!s->resident.empty()) {
       using clock = std::chrono::steady_clock;
       const auto us = [](clock::time_point a, clock::time_point b) {
@@ -755,6 +772,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
       const auto t_wait = clock::now();
       HIPCHECK(hipStreamSynchronize(s->stream));
       s->wait_us += us(t_wait, clock::now());
+      const auto t_retire = clock::now();
       std::vector<uint64_t> still;
       for (int i = 0; i < n; ++i) {
         Request &r = *s->requests.at(s->ctl_ids[i]);
@@ -782,6 +800,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
         s->resident.swap(keep);
         s->dirty = true;
       }
+      s->retire_us += us(t_retire, clock::now());
     }
     for (uint64_t id : done_now)
       s->finished.push_back(id);
@@ -872,6 +891,16 @@ int64_t wasl_hip_server_count(const wasl_hip_server *s, int what) {
     return s->wait_us;
   case 10:
     return s->prepare_us.load();
+  case 11:
+    return s->admit_us;
+  case 12:
+    return s->retire_us;
+  case 13:
+    return s->idle_us;
+  case 14:
+    return s->submit_us;
+  case 15:
+    return s->release_us;
   default:
     return -1;
   }

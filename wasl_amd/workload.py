@@ -219,7 +219,9 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
     makespan = time.perf_counter() - t0
     stats = {"rounds": s.count(6), "graphs": s.count(5), "resident_changes": s.count(7),
              "rebuild_s": s.count(8) * 1e-6, "round_wait_s": s.count(9) * 1e-6,
-             "prepare_s": s.count(10) * 1e-6}
+             "prepare_s": s.count(10) * 1e-6, "admit_s": s.count(11) * 1e-6,
+             "retire_s": s.count(12) * 1e-6, "idle_s": s.count(13) * 1e-6,
+             "submit_s": s.count(14) * 1e-6, "release_s": s.count(15) * 1e-6}
     s.close()
     return done, makespan, stats
 
