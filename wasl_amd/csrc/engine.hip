@@ -21,8 +21,13 @@
 #include "kernels.hpp"
 #include "setup.hpp"
 
+// A built mesh is immutable: engines and queued serving requests share it
+// instead of copying it.
 struct wasl_hip_mesh {
-  whip::Mesh m;
+  explicit wasl_hip_mesh(whip::Mesh built)
+      : shared(std::make_shared<const whip::Mesh>(std::move(built))), m(*shared) {}
+  std::shared_ptr<const whip::Mesh> shared;
+  const whip::Mesh &m;
 };
 
 namespace {
@@ -224,7 +229,7 @@ struct wasl_hip_engine {
   whip::PeriodicTrace trace;      // host copy for export
   whip::LimiterStencil stencil;   // host copy (built when limiting or on request)
   bool have_stencil = false;
-  whip::Mesh mesh;                // kept for stencil export and projection
+  std::shared_ptr<const whip::Mesh> mesh_ptr;   // kept for stencil export and projection
 
   // state and stage buffers
   Buffer u, w1, w2;
@@ -241,6 +246,7 @@ struct wasl_hip_engine {
 
   // timing of the last step/advance call
   std::vector<hipEvent_t> ev;
+  hipEvent_t wall[2] = {nullptr, nullptr};   // around wasl_hip_advance
   double last_wall_ms = 0.0, last_stage_ms = 0.0;
   int64_t last_stage_launches = 0;
   bool timing = false;
@@ -259,6 +265,9 @@ struct wasl_hip_engine {
           (void)hipGraphExecDestroy(x);
     for (auto e : ev)
       (void)hipEventDestroy(e);
+    for (auto e : wall)
+      if (e)
+        (void)hipEventDestroy(e);
     if (ctl_host)
       pinned_slots().put(ctl_host);
     if (result_host)
@@ -386,11 +395,11 @@ struct wasl_hip_engine {
     if (have_stencil || !euler())
       return;
     if (dim == 1) {
-      stencil = whip::line_limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC);
+      stencil = whip::line_limiter_stencil(*mesh_ptr, p, num.boundary_left == WASL_HIP_PERIODIC);
       have_stencil = true;
       return;
     }
-    stencil = whip::limiter_stencil(mesh, p, num.boundary_left == WASL_HIP_PERIODIC
+    stencil = whip::limiter_stencil(*mesh_ptr, p, num.boundary_left == WASL_HIP_PERIODIC
                                                  ? trace.donors
                                                  : std::vector<int32_t>{},
                                     p + 1);
@@ -865,8 +874,9 @@ void reset_device_error(wasl_hip_engine &g) {
 }
 
 void raise_device_error(wasl_hip_engine &g) {
-  int code[2] = {0, INT_MAX};
-  HIPCHECK(hipMemcpyAsync(code, g.err.p, sizeof code, hipMemcpyDeviceToHost, g.stream));
+  // read back through the engine's pinned slot (a pageable copy is slower)
+  int *code = reinterpret_cast<int *>(g.result_host);
+  HIPCHECK(hipMemcpyAsync(code, g.err.p, 2 * sizeof(int), hipMemcpyDeviceToHost, g.stream));
   HIPCHECK(hipStreamSynchronize(g.stream));
   if (code[0] != 0 || code[1] != INT_MAX) {
     reset_device_error(g);
@@ -1071,7 +1081,9 @@ EdgeRecords edge_records(const whip::Mesh &mesh, int p, int Np, const whip::Peri
   return r;
 }
 
-void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hip_engine &g) {
+void create_engine(std::shared_ptr<const whip::Mesh> shared, const wasl_hip_numerics &num,
+                   wasl_hip_engine &g) {
+  const whip::Mesh &mesh = *shared;
   g.num = num;
   g.p = num.polynomial_order;
   if (g.p < 0 || g.p > 3)
@@ -1079,7 +1091,7 @@ void create_engine(const whip::Mesh &mesh, const wasl_hip_numerics &num, wasl_hi
   if (num.precision != WASL_HIP_FP64 && num.precision != WASL_HIP_MIXED &&
       num.precision != WASL_HIP_BF16)
     throw std::runtime_error("engine: precision must be FP64, MIXED or BF16");
-  g.mesh = mesh;
+  g.mesh_ptr = std::move(shared);
   HIPCHECK(hipGetDevice(&g.device));
   g.stream = stream_pool().get(g.device);
   if (const char *env = std::getenv("WASL_HIP_GRAPHS"))
@@ -1361,6 +1373,8 @@ hipGraphExec_t step_graph(wasl_hip_engine &g, int k, int parity, whip::RunCtl *d
     HIPCHECK(hipStreamEndCapture(g.stream, &graph));
     HIPCHECK(hipGraphInstantiate(&x, graph, nullptr, nullptr, 0));
     HIPCHECK(hipGraphDestroy(graph));
+    // the one-time upload to the device happens here, not at the first replay
+    HIPCHECK(hipGraphUpload(x, g.stream));
   }
   return x;
 }
@@ -1457,9 +1471,8 @@ int wasl_hip_mesh_build(int dimension, int64_t node_count, const int64_t *node_t
   return guarded([&] {
     if (!out || !node_tags || !node_xyz || !element_nodes)
       throw std::runtime_error("mesh: null argument");
-    auto m = std::make_unique<wasl_hip_mesh>();
-    m->m = whip::build_mesh(dimension, node_count, node_tags, node_xyz, element_count,
-                            element_nodes);
+    auto m = std::make_unique<wasl_hip_mesh>(whip::build_mesh(
+        dimension, node_count, node_tags, node_xyz, element_count, element_nodes));
     *out = m.release();
   });
 }
@@ -1473,10 +1486,9 @@ int wasl_hip_mesh_from_prepared(int dimension, int64_t node_count, const int64_t
   return guarded([&] {
     if (!out)
       throw std::runtime_error("mesh: null argument");
-    auto m = std::make_unique<wasl_hip_mesh>();
-    m->m = whip::mesh_from_prepared(dimension, node_count, node_tags, points, element_count,
-                                    element_nodes, face_count, face_nodes, face_elements,
-                                    face_local_edges, face_normals);
+    auto m = std::make_unique<wasl_hip_mesh>(whip::mesh_from_prepared(
+        dimension, node_count, node_tags, points, element_count, element_nodes, face_count,
+        face_nodes, face_elements, face_local_edges, face_normals));
     *out = m.release();
   });
 }
@@ -1690,7 +1702,7 @@ int wasl_hip_engine_create(const wasl_hip_mesh *mesh, const wasl_hip_numerics *n
     if (!mesh || !numerics || !out)
       throw std::runtime_error("engine: null argument");
     auto g = std::make_unique<wasl_hip_engine>();
-    create_engine(mesh->m, *numerics, *g);
+    create_engine(mesh->shared, *numerics, *g);
     *out = g.release();
   });
 }
@@ -1765,7 +1777,7 @@ int wasl_hip_project(wasl_hip_engine *g, int ic, const double *params, int64_t p
       throw std::runtime_error("initial state: initial condition " + std::to_string(ic) +
                                " does not belong to this engine's equation");
     const std::vector<double> U =
-        whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, ic, params, param_count);
+        whip::project(*g->mesh_ptr, g->p, g->num.heat_capacity_ratio, ic, params, param_count);
     std::vector<double> tmp;
     HIPCHECK(hipMemcpyAsync(g->u.p, g->to_device(U.data(), tmp), U.size() * sizeof(double),
                             hipMemcpyHostToDevice, g->stream));
@@ -1956,9 +1968,11 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     double last_dt = 0.0;
     double stage_ms = 0.0;
     int64_t launches = 0;
-    hipEvent_t t0, t1;
-    HIPCHECK(hipEventCreate(&t0));
-    HIPCHECK(hipEventCreate(&t1));
+    if (!g->wall[0]) {
+      HIPCHECK(hipEventCreate(&g->wall[0]));
+      HIPCHECK(hipEventCreate(&g->wall[1]));
+    }
+    hipEvent_t t0 = g->wall[0], t1 = g->wall[1];
     HIPCHECK(hipEventRecord(t0, g->stream));
     while (true) {
       // Steps beyond max_steps would only be skipped on the device; do not
@@ -2008,8 +2022,6 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
     HIPCHECK(hipEventSynchronize(t1));
     float wall = 0.0f;
     HIPCHECK(hipEventElapsedTime(&wall, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     g->last_wall_ms = wall;
     g->last_stage_ms = stage_ms;
     g->last_stage_launches = launches;
@@ -2019,7 +2031,8 @@ int wasl_hip_advance(wasl_hip_engine *g, double cfl, double end_time, uint64_t m
       HIPCHECK(hipStreamSynchronize(g->stream));
       throw SimulationError(device_error_text(code));
     }
-    raise_device_error(*g);
+    // Inside the loop the kernels report through the step control (h->err,
+    // read above); the engine's own error word is only written outside it.
     *time = cur_time;
     *step = cur_step;
     if (history_rows)
@@ -2120,6 +2133,16 @@ int wasl_hip_capture_graphs(wasl_hip_engine *g) {
     for (int k = 0; k < wasl_hip_engine::kGraphLevels; ++k)
       for (int parity = 0; parity < 2; ++parity)
         step_graph(*g, k, parity, g->ctl.as<whip::RunCtl>());
+    // One replay of each graph with a step control whose run is already over
+    // (time 0, end time 0): every stage returns at once and the state is not
+    // touched, but the first-replay costs are paid here. The next advance
+    // writes a fresh step control.
+    std::memset(g->ctl_host, 0, sizeof(whip::RunCtl));
+    HIPCHECK(hipMemcpyAsync(g->ctl.p, g->ctl_host, sizeof(whip::RunCtl), hipMemcpyHostToDevice,
+                            g->stream));
+    for (auto &level : g->graphs)
+      for (hipGraphExec_t x : level)
+        HIPCHECK(hipGraphLaunch(x, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
   });
 }

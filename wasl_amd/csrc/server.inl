@@ -19,7 +19,7 @@ enum class Phase { queued, preparing, ready, resident, done };
 
 struct Request {
   uint64_t id = 0;
-  whip::Mesh mesh;
+  std::shared_ptr<const whip::Mesh> mesh;
   wasl_hip_numerics num{};
   int ic = 0;
   std::vector<double> params;
@@ -173,7 +173,7 @@ void prepare(Request &r) {
     auto g = std::make_unique<wasl_hip_engine>();
     create_engine(r.mesh, r.num, *g);
     const std::vector<double> U =
-        whip::project(g->mesh, g->p, g->num.heat_capacity_ratio, r.ic, r.params.data(),
+        whip::project(*g->mesh_ptr, g->p, g->num.heat_capacity_ratio, r.ic, r.params.data(),
                       static_cast<int64_t>(r.params.size()));
     HIPCHECK(hipMemcpyAsync(g->u.p, U.data(), U.size() * sizeof(double), hipMemcpyHostToDevice,
                             g->stream));
@@ -190,6 +190,13 @@ void prepare(Request &r) {
       r.end_time = r.end_time > 0.0 ? std::min(by_steps, r.end_time) : by_steps;
     }
     HIPCHECK(hipStreamSynchronize(g->stream));
+    // A served engine exports nothing: its host copies of the stencil and the
+    // traces (the transforms alone reach hundreds of MB) go now, on this
+    // worker, rather than when the pump thread retires the case.
+    std::vector<double>().swap(g->stencil.transforms);
+    std::vector<int32_t>().swap(g->stencil.neighbors);
+    std::vector<int32_t>().swap(g->trace.donors);
+    std::vector<double>().swap(g->trace.basis);
     r.eng = std::move(g);
   } catch (const SimulationError &ex) {
     r.status = WASL_HIP_SIMULATION;
@@ -647,9 +654,9 @@ int wasl_hip_server_submit(wasl_hip_server *s, const wasl_hip_mesh *mesh,
           "server: serving takes 2D Euler and 1D scalar advection cases; run 1D Euler and 2D "
           "advection on their own engines");
     auto r = std::make_unique<Request>();
-    r->mesh = mesh->m;
+    r->mesh = mesh->shared;
     r->num = *numerics;
-    const bool fp64 = r->mesh.dimension != 2 ||
+    const bool fp64 = r->mesh->dimension != 2 ||
                       r->num.polynomial_order <= s->cfg.fp64_max_order ||
                       (s->cfg.fp64_limited && r->num.limiting);
     r->num.precision = fp64 ? WASL_HIP_FP64 : s->cfg.precision;
@@ -694,7 +701,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
           ++active;
       while (!s->queue.empty() && active < static_cast<size_t>(s->cfg.max_resident)) {
         Request &r = *s->requests.at(s->queue.front());
-        const size_t need = estimate_bytes(r.mesh, r.num);
+        const size_t need = estimate_bytes(*r.mesh, r.num);
         if (s->reserved + need > s->budget && s->reserved > 0)
           break;
         s->queue.pop_front();
