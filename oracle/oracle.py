@@ -363,3 +363,68 @@ def line_scalar(node_tags, node_x, segments, order, wavespeed, left=(PERIODIC, 0
     if not h:
         raise OracleError(lib.orc_error().decode())
     return Case(lib, h, 1)
+
+
+class ReferenceIO:
+    """The reference's prepared-mesh and checkpoint files through
+    oracle/_ref/libwaslref_io.so (oracle/ref_io.cpp); test infrastructure."""
+
+    PATH = os.path.join(HERE, "_ref", "libwaslref_io.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        lib = C.CDLL(self.PATH)
+        lib.orc_io_error.restype = C.c_char_p
+        lib.orc_io_prepared_write.argtypes = [_i64, _pi64, _pd, _i64, _pi64, C.c_char_p]
+        lib.orc_io_prepared_copy.argtypes = [C.c_char_p, C.c_char_p, _pi64]
+        lib.orc_io_checkpoint_write.argtypes = [C.c_char_p, _d, C.c_uint64, C.c_uint64,
+                                                C.c_uint64, C.c_uint64, C.c_char_p, _pd]
+        lib.orc_io_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(_d), _pu64,
+                                               np.ctypeslib.ndpointer(np.uint64), _P, _i64,
+                                               C.c_char_p, _i64]
+        for name in ("orc_io_prepared_write", "orc_io_prepared_copy", "orc_io_checkpoint_write",
+                     "orc_io_checkpoint_read"):
+            getattr(lib, name).restype = C.c_int
+        self.lib = lib
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OracleError(self.lib.orc_io_error().decode())
+
+    def write_prepared(self, node_tags, node_xy, triangles, path):
+        tags = np.ascontiguousarray(node_tags, np.int64)
+        xy = np.ascontiguousarray(node_xy, np.float64).ravel()
+        tri = np.ascontiguousarray(triangles, np.int64).ravel()
+        self._check(self.lib.orc_io_prepared_write(tags.size, tags, xy, tri.size // 3, tri,
+                                                   str(path).encode()))
+
+    def copy_prepared(self, src, dst=""):
+        """Reads src with the reference (and writes it to dst when given);
+        returns (dimension, nodes, elements, faces)."""
+        counts = np.zeros(4, np.int64)
+        self._check(self.lib.orc_io_prepared_copy(str(src).encode(), str(dst).encode(), counts))
+        return tuple(int(x) for x in counts)
+
+    def write_checkpoint(self, path, time, step, V, Np, E, prepared_mesh, coefficients):
+        c = np.ascontiguousarray(coefficients, np.float64).ravel()
+        self._check(self.lib.orc_io_checkpoint_write(str(path).encode(), time, step, V, Np, E,
+                                                     str(prepared_mesh).encode(), c))
+
+    def read_checkpoint(self, path):
+        t = _d()
+        s = C.c_uint64()
+        layout = np.zeros(4, np.uint64)
+        name = C.create_string_buffer(4096)
+        self._check(self.lib.orc_io_checkpoint_read(str(path).encode(), C.byref(t), C.byref(s),
+                                                    layout, None, 0, name, 4096))
+        n = int(layout[0] * layout[1] * layout[2])
+        coeff = np.zeros(n)
+        self._check(self.lib.orc_io_checkpoint_read(str(path).encode(), None, None, layout,
+                                                    coeff.ctypes.data, n, None, 0))
+        return {"time": t.value, "step": s.value, "variable_count": int(layout[0]),
+                "mode_count": int(layout[1]), "element_count": int(layout[2]),
+                "partition_rank_count": int(layout[3]), "prepared_mesh": name.value.decode(),
+                "coefficients": coeff}

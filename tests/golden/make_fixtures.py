@@ -6,6 +6,8 @@ the reference is present: `python tests/golden/make_fixtures.py`).
                             the reference's stored case-regression metrics
                             (cpp/tests/runtime/case_regression_reference.txt),
                             copied verbatim as data;
+  ref_prepared_box.h5,      a prepared mesh and a checkpoint written by the
+  ref_checkpoint_p1.h5      reference's own HDF5 writers (format checks);
   reference_vectors.npz     inputs and outputs of the reference's own code
                             (oracle/_ref, compiled from the reference sources)
                             on small seeded meshes: residuals P0-P3, limiter
@@ -190,7 +192,38 @@ def positivity_vectors(out) -> None:
     out["floor_error_p2_partial"] = part
 
 
+# On-disk formats written by the reference's own writers (oracle/_ref's
+# libwaslref_io.so): a prepared mesh of a 2x4x4 jittered periodic box and a
+# checkpoint of the projected P1 vortex on it (the mesh path stored as the
+# file name alone).
+FORMAT_MESH = "ref_prepared_box.h5"
+FORMAT_CHECKPOINT = "ref_checkpoint_p1.h5"
+
+
+def format_checkpoint_fields():
+    import cases as K
+    from oracle import oracle as O
+
+    c = K.oracle_tri(K.vortex_mesh(4), 1)
+    return dict(time=0.125, step=42, variable_count=4, mode_count=3, element_count=c.E,
+                prepared_mesh=FORMAT_MESH, coefficients=c.project(O.IC_VORTEX, K.VORTEX))
+
+
+def formats() -> None:
+    import cases as K
+    from oracle import oracle as O
+
+    io = O.ReferenceIO()
+    m = K.vortex_mesh(4)
+    io.write_prepared(m.node_tags, m.node_xy, m.triangles, os.path.join(HERE, FORMAT_MESH))
+    f = format_checkpoint_fields()
+    io.write_checkpoint(os.path.join(HERE, FORMAT_CHECKPOINT), f["time"], f["step"],
+                        f["variable_count"], f["mode_count"], f["element_count"],
+                        f["prepared_mesh"], f["coefficients"])
+
+
 if __name__ == "__main__":
     regression_rows()
     vectors()
+    formats()
     print("fixtures written to", HERE)

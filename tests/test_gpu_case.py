@@ -113,3 +113,45 @@ def test_runner_outputs_follow_reference_cadence(gpu, tmp_path):
     assert cks == [f"checkpoint-{k:06d}.h5" for k in written[-2:]]
     assert s.endswith(f"checkpoint={out / f'checkpoint-{step:06d}.h5'}")
     assert (out / "solution.pvd").read_text().count("<DataSet") == len(vtus)
+
+
+def test_runs_on_reference_files(gpu, tmp_path):
+    """A case run entirely on files of the reference's own formats: the mesh
+    is written by the reference's prepare_mesh and writer, the run's
+    checkpoint is read by the reference's reader, and a restart from a
+    checkpoint the reference wrote (same fields, its own writer) ends where
+    the uninterrupted run ends."""
+    from oracle import oracle as O
+    from wasl_amd import case
+
+    if not O.ReferenceIO.available():
+        pytest.skip("oracle/_ref/libwaslref_io.so is not built here (no /root/reference)")
+    import cases as K
+
+    io = O.ReferenceIO()
+    m = K.vortex_mesh(6)
+    mesh_path = tmp_path / "box.h5"
+    io.write_prepared(m.node_tags, m.node_xy, m.triangles, mesh_path)
+    regions = [0, 0, 0, 0, 0, 0, 1.0, 0.3, 0.2, 1.0,
+               2, 0, 0, 5.0, 5.0, 2.0, 0.5, 0.0, 0.0, 0.7]   # a limited disc
+    args = (True, 1e-3, 1e-10, 1e-10, 2, 2, 1.4, 0.2)
+
+    def run(out, restart):
+        return dict(kv.split("=", 1) for kv in case.run_euler_case(
+            mesh_path, out, restart, *args, 0.0, 0.2, 10 ** 6, 1, 10 ** 6, 5, 100,
+            regions=regions).split(" "))
+
+    full = run(tmp_path / "full", "")
+    mid = tmp_path / "full" / "checkpoint-000005.h5"
+    ck = io.read_checkpoint(mid)
+    assert ck["step"] == 5 and ck["element_count"] == 72 and ck["partition_rank_count"] == 1
+    assert os.path.samefile(ck["prepared_mesh"], mesh_path)
+    final = io.read_checkpoint(full["checkpoint"])
+    assert final["step"] == int(full["step"]) > 5 and final["time"] == float(full["time"]) == 0.2
+    theirs = tmp_path / "reference_checkpoint.h5"
+    io.write_checkpoint(theirs, ck["time"], ck["step"], ck["variable_count"], ck["mode_count"],
+                        ck["element_count"], ck["prepared_mesh"], ck["coefficients"])
+    resumed = run(tmp_path / "resumed", str(theirs))
+    assert resumed["step"] == full["step"] and resumed["time"] == full["time"]
+    b = io.read_checkpoint(resumed["checkpoint"])["coefficients"]
+    assert np.array_equal(final["coefficients"], b)
