@@ -385,8 +385,12 @@ class ReferenceIO:
         lib.orc_io_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(_d), _pu64,
                                                np.ctypeslib.ndpointer(np.uint64), _P, _i64,
                                                C.c_char_p, _i64]
+        lib.orc_io_export.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        lib.orc_io_write_history.argtypes = [C.c_char_p, _i64, _pd]
+        lib.orc_io_write_pvd.argtypes = [C.c_char_p, _i64, _pd, C.c_char_p]
         for name in ("orc_io_prepared_write", "orc_io_prepared_copy", "orc_io_checkpoint_write",
-                     "orc_io_checkpoint_read"):
+                     "orc_io_checkpoint_read", "orc_io_export", "orc_io_write_history",
+                     "orc_io_write_pvd"):
             getattr(lib, name).restype = C.c_int
         self.lib = lib
 
@@ -428,3 +432,18 @@ class ReferenceIO:
                 "mode_count": int(layout[1]), "element_count": int(layout[2]),
                 "partition_rank_count": int(layout[3]), "prepared_mesh": name.value.decode(),
                 "coefficients": coeff}
+
+    def export(self, checkpoint_path, mesh_path, out):
+        """The bridge's export_checkpoint with the reference's VTU writer."""
+        self._check(self.lib.orc_io_export(str(checkpoint_path).encode(),
+                                           str(mesh_path).encode(), str(out).encode()))
+
+    def write_history(self, path, rows):
+        r = np.ascontiguousarray(rows, np.float64).reshape(-1, 4)
+        self._check(self.lib.orc_io_write_history(str(path).encode(), r.shape[0], r.ravel()))
+
+    def write_pvd(self, path, snapshots):
+        times = np.ascontiguousarray([t for t, _ in snapshots], np.float64)
+        names = "\n".join(n for _, n in snapshots).encode()
+        self._check(self.lib.orc_io_write_pvd(str(path).encode(), len(snapshots),
+                                              times if times.size else np.zeros(1), names))

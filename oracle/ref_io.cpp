@@ -7,13 +7,19 @@
 // engine's case layer are read by the reference and the other way round.
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <sstream>
 #include <string>
+#include <vector>
 
 #include "wasl/io/checkpoint.hpp"
 #include "wasl/io/checkpoint_file.hpp"
+#include "wasl/io/history_csv.hpp"
 #include "wasl/io/prepared_mesh_file.hpp"
+#include "wasl/io/pvd_writer.hpp"
+#include "wasl/io/vtu_writer.hpp"
 #include "wasl/mesh/msh/msh_document.hpp"
 #include "wasl/mesh/prepared/prepare_mesh.hpp"
 #include "wasl/mesh/triangle/build_triangle_mesh.hpp"
@@ -112,6 +118,48 @@ int orc_io_checkpoint_read(const char *path, double *time, uint64_t *step, uint6
       std::memcpy(mesh_path, c.prepared_mesh.data(), n);
       mesh_path[n] = '\0';
     }
+  });
+}
+
+// The bridge's export_checkpoint (crates/wasl-bridge/src/bridge.cpp:107-126):
+// cell averages c_0 / sqrt(2) of a checkpoint on its prepared mesh, written by
+// the reference's write_vtu.
+int orc_io_export(const char *checkpoint_path, const char *mesh_path, const char *out) {
+  return guarded([&] {
+    const auto c = wasl::read_checkpoint(checkpoint_path);
+    const auto mesh = wasl::read_prepared_mesh(mesh_path);
+    if (c.element_count != mesh.element_count())
+      throw std::runtime_error("checkpoint export: mesh element count mismatch");
+    const double inverse_sqrt2 = 1.0 / std::sqrt(2.0);
+    std::vector<double> averages(c.variable_count * c.element_count);
+    for (std::size_t v = 0; v < c.variable_count; ++v)
+      for (std::size_t e = 0; e < c.element_count; ++e)
+        averages[e * c.variable_count + v] =
+            c.coefficients[(v * c.mode_count) * c.element_count + e] * inverse_sqrt2;
+    wasl::write_vtu(out, mesh, averages, c.variable_count);
+  });
+}
+
+// write_csv_history of rows [n][4] = (step, time, dt, mass).
+int orc_io_write_history(const char *path, int64_t n, const double *rows) {
+  return guarded([&] {
+    std::vector<wasl::HistoryRow> h;
+    for (int64_t i = 0; i < n; ++i)
+      h.push_back({static_cast<std::uint64_t>(rows[4 * i]), rows[4 * i + 1], rows[4 * i + 2],
+                   rows[4 * i + 3]});
+    wasl::write_csv_history(path, h);
+  });
+}
+
+// write_pvd of n snapshots: times[n] and the file names joined by newlines.
+int orc_io_write_pvd(const char *path, int64_t n, const double *times, const char *names) {
+  return guarded([&] {
+    std::vector<std::pair<double, std::string>> snaps;
+    std::istringstream in(names);
+    std::string name;
+    for (int64_t i = 0; i < n && std::getline(in, name); ++i)
+      snaps.emplace_back(times[i], name);
+    wasl::write_pvd(path, snaps);
   });
 }
 

@@ -110,3 +110,16 @@ def test_reference_reads_engine_checkpoint(case, reference_io, tmp_path):
                                   got["coefficients"])
     again = case.read_checkpoint(back)
     assert again["step"] == 123 and np.array_equal(again["coefficients"], U)
+
+
+def test_exported_vtu_is_the_reference_bytes(case, reference_io, tmp_path):
+    """export_checkpoint of the reference's checkpoint on the reference's
+    mesh: the engine's VTU file and the reference's (bridge.cpp:107-126 with
+    its write_vtu) are byte for byte the same."""
+    from golden.make_fixtures import FORMAT_CHECKPOINT, FORMAT_MESH
+
+    ck, mesh = os.path.join(GOLDEN, FORMAT_CHECKPOINT), os.path.join(GOLDEN, FORMAT_MESH)
+    ours, theirs = tmp_path / "engine.vtu", tmp_path / "reference.vtu"
+    assert case.export_checkpoint(ck, mesh, ours) == "exported 32 cells"
+    reference_io.export(ck, mesh, theirs)
+    assert ours.read_bytes() == theirs.read_bytes()

@@ -138,7 +138,7 @@ def test_runs_on_reference_files(gpu, tmp_path):
 
     def run(out, restart):
         return dict(kv.split("=", 1) for kv in case.run_euler_case(
-            mesh_path, out, restart, *args, 0.0, 0.2, 10 ** 6, 1, 10 ** 6, 5, 100,
+            mesh_path, out, restart, *args, 0.0, 0.2, 10 ** 6, 1, 5, 5, 100,
             regions=regions).split(" "))
 
     full = run(tmp_path / "full", "")
@@ -155,3 +155,19 @@ def test_runs_on_reference_files(gpu, tmp_path):
     assert resumed["step"] == full["step"] and resumed["time"] == full["time"]
     b = io.read_checkpoint(resumed["checkpoint"])["coefficients"]
     assert np.array_equal(final["coefficients"], b)
+
+    # the run's text outputs are the bytes the reference's writers emit for
+    # the same rows, snapshots and cell averages
+    import re
+
+    out = tmp_path / "full"
+    rows = np.loadtxt(out / "history.csv", delimiter=",", skiprows=1, ndmin=2)
+    io.write_history(tmp_path / "history.csv", rows)
+    assert (out / "history.csv").read_bytes() == (tmp_path / "history.csv").read_bytes()
+    pvd = (out / "solution.pvd").read_text()
+    snaps = [(float(t), f) for t, f in re.findall(r'timestep="([^"]+)"[^>]*file="([^"]+)"', pvd)]
+    assert [f for _, f in snaps[:2]] == ["solution-000000.vtu", "solution-000005.vtu"]
+    io.write_pvd(tmp_path / "solution.pvd", snaps)
+    assert (out / "solution.pvd").read_bytes() == (tmp_path / "solution.pvd").read_bytes()
+    io.export(mid, mesh_path, tmp_path / "step5.vtu")
+    assert (out / "solution-000005.vtu").read_bytes() == (tmp_path / "step5.vtu").read_bytes()
