@@ -446,6 +446,10 @@ std::vector<double> euler_tables(int p) {
       t[q * Np + m] = b[m];
       t[kWx + q * Np + m] = vol.w[q] * gx[m];
       t[kWy + q * Np + m] = vol.w[q] * ge[m];
+      // the stage kernels leave these products out
+      if ((whip::dubiner_dxi_zero(m) && gx[m] != 0.0) ||
+          (whip::dubiner_deta_zero(m) && ge[m] != 0.0))
+        throw std::runtime_error("engine: a Dubiner gradient entry taken as zero is not");
     }
   }
   edge_tables(p, t.data() + kBl, t.data() + kBr, t.data() + kWe);

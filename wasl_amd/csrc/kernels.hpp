@@ -46,6 +46,19 @@ template <int P> struct Tri {
   static constexpr int kEdgeTable = kTable - kBl;
 };
 
+// Dubiner modes are ordered by degree n, then by i (basis/dubiner_triangle.hpp:
+// 20-23). A mode with i = 0 depends on eta alone, so its xi-derivative is
+// identically zero, and mode 0 is constant. The weighted-gradient tables hold
+// exact zeros there (the host checks when it builds them), and the volume
+// contractions leave those products out.
+__host__ __device__ constexpr bool dubiner_dxi_zero(int m) {
+  int n = 0;
+  while ((n + 1) * (n + 2) / 2 <= m)
+    ++n;
+  return m == n * (n + 1) / 2;
+}
+__host__ __device__ constexpr bool dubiner_deta_zero(int m) { return m == 0; }
+
 template <int P> struct Line {
   static constexpr int Np = P + 1;
   static constexpr int NQ = P > 0 ? P : 1;
@@ -531,18 +544,23 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
         const double inv = rcp(s.r);
         const double u = s.mx * inv, w = s.my * inv;
         const double p = g1 * (s.E - 0.5 * (s.mx * u + s.my * w));
-        const double fx[4] = {s.mx, s.mx * u + p, s.my * u, (s.E + p) * u};
-        const double fy[4] = {s.my, s.mx * w, s.my * w + p, (s.E + p) * w};
+        // reference-frame fluxes through the contravariant velocities
+        // U_xi = A u + B v, U_eta = C u + D v
+        const double uxi = gA * u + gB * w, ueta = gC * u + gD * w, enp = s.E + p;
+        const double fxi[4] = {s.r * uxi, s.mx * uxi + gA * p, s.my * uxi + gB * p, enp * uxi};
+        const double feta[4] = {s.r * ueta, s.mx * ueta + gC * p, s.my * ueta + gD * p,
+                                enp * ueta};
         const double *wx = tab + D::kWx + q * Np;
         const double *wy = tab + D::kWy + q * Np;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const double fxi = gA * fx[v] + gB * fy[v];
-          const double feta = gC * fx[v] + gD * fy[v];
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
-          for (int m = 0; m < Np; ++m)
-            acc[v][m] += wx[m] * fxi + wy[m] * feta;
-        }
+          for (int m = 0; m < Np; ++m) {
+            if (!dubiner_dxi_zero(m))
+              acc[v][m] = fma(wx[m], fxi[v], acc[v][m]);
+            if (!dubiner_deta_zero(m))
+              acc[v][m] = fma(wy[m], feta[v], acc[v][m]);
+          }
       }
     }
   }
@@ -934,8 +952,12 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
         const double *wx = vt + D::kVWx + (q0 + j) * Np;
         const double *wy = vt + D::kVWy + (q0 + j) * Np;
 #pragma unroll
-        for (int m = 0; m < Np; ++m)
-          acc[m] += wx[m] * fxi + wy[m] * feta;
+        for (int m = 0; m < Np; ++m) {
+          if (!dubiner_dxi_zero(m))
+            acc[m] = fma(wx[m], fxi, acc[m]);
+          if (!dubiner_deta_zero(m))
+            acc[m] = fma(wy[m], feta, acc[m]);
+        }
       }
       lane_exchange_fence();
     }
