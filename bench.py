@@ -491,7 +491,7 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
 
     tts = [done_o[k][1] - done_o[k][0] for k in both]
     plain_tts = [mix[k].plain_done - arrivals[k] for k in both]
-    worst = {}
+    worst, worst_coef = {}, {}
     for k in both:
         r = mix[k]
         if r.kind == "advection":
@@ -499,7 +499,9 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         Np = (r.order + 1) * (r.order + 2) // 2
         a = done[k][3].reshape(4, Np, r.size)[0]
         b = r.plain_state.reshape(4, Np, r.size)[0]
-        worst[r.kind] = max(worst.get(r.kind, 0.0), float(np.max(np.abs(a - b))))
+        worst_coef[r.kind] = max(worst_coef.get(r.kind, 0.0), float(np.max(np.abs(a - b))))
+        worst[r.kind] = max(worst.get(r.kind, 0.0),
+                            WL.density_field_linf(done[k][3], r.plain_state, r.order, r.size))
     out = {
         "requests": count,
         "requests_measured": len(both),
@@ -531,8 +533,11 @@ def serving_measurement(count: int, seed: int, steps_per_batch: int = 16,
         "plain_open_loop_wall_s": plain_wall,
         "closed_loop_server": stats,
         "density_linf_served_vs_fp64": worst,
-        "density_linf_note": "max over requests of each kind of |rho coefficients, served "
-                             "(mixed) - plain (FP64)|; limited cases run the FP64 stage",
+        "density_linf_note": "max over requests of each kind of |rho_served(x) - rho_plain(x)|, "
+                             "the density fields (mixed against FP64) sampled at the "
+                             "(2p+3)(2p+4)/2 lattice points of every element; limited cases run "
+                             "the FP64 stage",
+        "density_coefficient_linf_served_vs_fp64": worst_coef,
     }
     log(f"TTS p50 {out['tts_p50_s']:.3f} s p99 {out['tts_p99_s']:.3f} s (plain p50 "
         f"{out['plain_tts_p50_s']:.3f} s p99 {out['plain_tts_p99_s']:.3f} s); density L-inf "

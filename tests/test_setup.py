@@ -176,3 +176,19 @@ def test_server_config_defaults(wasl):
     assert (c.max_resident, c.steps_per_batch, c.setup_threads) == (256, 16, 4)
     assert (c.fp64_max_order, c.fp64_limited, c.use_graphs) == (1, 1, 1)
     assert ctypes.sizeof(c) == 72   # sizeof(wasl_hip_server_config) on x86-64
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_density_lattice_basis_equals_oracle(order):
+    """The serving bench samples density fields through its own Dubiner
+    evaluation; it agrees with the oracle's basis at every lattice point,
+    vertices (the collapsed top vertex included) and edges too."""
+    from wasl_amd import workload as WL
+
+    R = O.Routines()
+    n = 2 * order + 2
+    pts = [(-1.0 + 2.0 * a / n, -1.0 + 2.0 * b / n) for b in range(n + 1) for a in range(n + 1 - b)]
+    ref = np.array([R.dubiner(x, y, order) for x, y in pts])
+    B = WL.dubiner_lattice(order)
+    assert B.shape == ref.shape == (len(pts), (order + 1) * (order + 2) // 2)
+    np.testing.assert_allclose(B, ref, rtol=1e-13, atol=1e-13)

@@ -435,8 +435,8 @@ void edge_tables(int p, double *bl, double *br, double *we) {
 std::vector<double> euler_tables(int p) {
   const int Np = whip::tri_modes(p), NQ = (p + 3) * (p + 3), NE = p + 1;
   const int kWx = NQ * Np, kWy = 2 * NQ * Np, kBl = 3 * NQ * Np;
-  const int kBr = kBl + 3 * NE * Np, kWe = kBr + 18 * NE * Np;
-  std::vector<double> t(kWe + 3 * NE, 0.0);
+  const int kBr = kBl + 3 * NE * Np, kWe = kBr + 18 * NE * Np, kCs = kWe + 3 * NE;
+  std::vector<double> t(kCs + 2 * Np, 0.0);
   const whip::TriRule vol = whip::triangle_rule(p + 2);
   std::vector<double> b(Np), gx(Np), ge(Np);
   for (int q = 0; q < NQ; ++q) {
@@ -452,6 +452,12 @@ std::vector<double> euler_tables(int p) {
         throw std::runtime_error("engine: a Dubiner gradient entry taken as zero is not");
     }
   }
+  // column sums of the weighted gradients (the volume term of a constant flux)
+  for (int m = 0; m < Np; ++m)
+    for (int q = 0; q < NQ; ++q) {
+      t[kCs + m] += t[kWx + q * Np + m];
+      t[kCs + Np + m] += t[kWy + q * Np + m];
+    }
   edge_tables(p, t.data() + kBl, t.data() + kBr, t.data() + kWe);
   return t;
 }
@@ -538,9 +544,11 @@ template <int P> std::vector<float> mixed_tables() {
   for (int l = 0; l < 64; ++l) {
     const int i = l & 15, kk = l >> 4;
     for (int tt = 0; tt < MT; ++tt) {
+      // the states at points from the higher modes only (the kernel works
+      // with deviations from the cell average)
       for (int s = 0; s < KS; ++s)
         t[X::kAvol + (tt * KS + s) * 64 + l] =
-            static_cast<float>(vb(X::point(tt, i), KS * kk + s));
+            KS * kk + s == 0 ? 0.0f : static_cast<float>(vb(X::point(tt, i), KS * kk + s));
       // K step r of tile tt takes point X::point(tt, 4 kk + r) from row group kk
       for (int r = 0; r < 4; ++r)
         for (int d = 0; d < 2; ++d)
@@ -550,7 +558,8 @@ template <int P> std::vector<float> mixed_tables() {
       // volume product K entry 4 kk + c is mode KS kk + c (c < KS)
       uint16_t hv[4], hx[4], he[4];
       for (int c = 0; c < 4; ++c) {
-        hv[c] = bf16_bits(static_cast<float>(c < KS ? vb(X::point(tt, i), KS * kk + c) : 0.0));
+        hv[c] = bf16_bits(static_cast<float>(c < KS && KS * kk + c != 0
+                                                 ? vb(X::point(tt, i), KS * kk + c) : 0.0));
         hx[c] = bf16_bits(static_cast<float>(wd(0, X::point(tt, 4 * kk + c), X::row_mode(i))));
         he[c] = bf16_bits(static_cast<float>(wd(1, X::point(tt, 4 * kk + c), X::row_mode(i))));
       }

@@ -34,6 +34,11 @@ template <int P> struct Tri {
   static constexpr int kBr = kBl + 3 * NE * Np;          // [3][3][2][NE][Np]
   static constexpr int kWe = kBr + 18 * NE * Np;         // [3][NE]
   static constexpr int kTable = kWe + 3 * NE;
+  // Column sums of the weighted gradients, sum_q W_xi[q][m] and sum_q
+  // W_eta[q][m] ([2][Np]): the volume term of a constant flux, used by the
+  // mixed-precision kernel (uploaded after the staged table).
+  static constexpr int kCs = kTable;
+  static constexpr int kFull = kCs + 2 * Np;
   // Volume tables padded to a multiple of 8 points (zero weights) for the
   // variable-split kernel: B [NQ8][Np], W_xi [NQ8][Np], W_eta [NQ8][Np].
   static constexpr int NQ8 = (NQ + 7) / 8 * 8;
@@ -1203,10 +1208,12 @@ typedef float mx_f4 __attribute__((ext_vector_type(4)));
 
 // Waves per SIMD the mixed stage kernels are built for (register budget 512 /
 // waves): every global read sits in registers from the start of the kernel.
-// Measured on MI355X: five waves at P1 and four at P2 beat three (7-14 %);
-// P3 needs the registers of three.
+// The deviation-form volume holds the cell average's state through the
+// volume products; P1 fits 128 registers (four waves) and P2 and P3 168
+// (three waves). Measured on MI355X at P2: three waves without spills beat
+// four with a 20-byte spill (155.9 against 157.5 us at 524288 triangles).
 template <int P, bool BF16> constexpr int mx_waves() {
-  return P <= 1 ? 5 : P == 2 ? 4 : 3;
+  return P == 0 ? 5 : P == 1 ? 4 : 3;
 }
 typedef short mx_s4 __attribute__((ext_vector_type(4)));
 
@@ -1394,8 +1401,33 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     R[v] = mx_f4{0.0f, 0.0f, 0.0f, 0.0f};
 
   // ---- volume: states at points (one product per variable, shared A),
-  // pointwise fluxes once per point, flux contraction
+  // pointwise fluxes once per point, flux contraction. The products run on
+  // deviations from the cell average: the states at points come from the
+  // higher modes only (the operand table leaves mode 0 out), and a point's
+  // flux enters the contraction as its deviation from the flux of the
+  // average, formed from the deviations so that FP32 rounds small numbers.
+  // The constant flux's own volume term is added back in FP64 in the
+  // epilogue (column sums of the weighted gradients).
+  constexpr double kPhi0 = 0.70710678118654752440;   // mode 0 on the reference triangle
   if constexpr (P > 0) {
+    float br, bmx, bmy, bu, bv, bh, buxi, bueta;
+    {
+      double m[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        m[v] = __shfl(c64[v][0], j, 64) * kPhi0;
+      const double inv = rcp(m[0]);
+      const double u = m[1] * inv, w = m[2] * inv;
+      const double pb = (mesh.gamma - 1.0) * (m[3] - 0.5 * (m[1] * u + m[2] * w));
+      br = static_cast<float>(m[0]);
+      bmx = static_cast<float>(m[1]);
+      bmy = static_cast<float>(m[2]);
+      bu = static_cast<float>(u);
+      bv = static_cast<float>(w);
+      bh = static_cast<float>(m[3] + pb);
+      buxi = gA * bu + gB * bv;
+      bueta = gC * bu + gD * bv;
+    }
     mx_s4 cb16[4];
     if constexpr (BF16) {
 #pragma unroll
@@ -1430,15 +1462,21 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
         if (!BF16 && !X::step_real(t, r))
           continue;   // no real point in this K step: the step is not issued
         const bool real = X::point(t, 4 * grp + r) < NQ;   // padded points: zero fluxes
-        const float rho = S[0][r], mx = S[1][r], my = S[2][r], en = S[3][r];
-        const float inv = rcpf(real ? rho : 1.0f);
-        const float u = mx * inv, w = my * inv;
-        const float p = g1 * (en - 0.5f * (mx * u + my * w));
-        // reference-frame fluxes through the contravariant velocities
-        // U_xi = A u + B v, U_eta = C u + D v
-        const float uxi = gA * u + gB * w, ueta = gC * u + gD * w, enp = en + p;
-        const float fxi[4] = {rho * uxi, mx * uxi + gA * p, my * uxi + gB * p, enp * uxi};
-        const float feta[4] = {rho * ueta, mx * ueta + gC * p, my * ueta + gD * p, enp * ueta};
+        // deviations (r1, x1, y1, e1) of density, momenta and energy
+        const float r1 = S[0][r], x1 = S[1][r], y1 = S[2][r], e1 = S[3][r];
+        const float inv = rcpf(real ? br + r1 : 1.0f);
+        const float du = (x1 - bu * r1) * inv, dv = (y1 - bv * r1) * inv;
+        const float u = bu + du, w = bv + dv;
+        const float dk = 0.5f * ((bmx * du + x1 * u) + (bmy * dv + y1 * w));
+        const float dp = g1 * (e1 - dk), de = e1 + dp;
+        // reference-frame flux deviations through the contravariant
+        // velocities U_xi = A u + B v, U_eta = C u + D v and their deviations
+        const float duxi = gA * du + gB * dv, dueta = gC * du + gD * dv;
+        const float uxi = buxi + duxi, ueta = bueta + dueta;
+        const float fxi[4] = {br * duxi + r1 * uxi, bmx * duxi + x1 * uxi + gA * dp,
+                              bmy * duxi + y1 * uxi + gB * dp, bh * duxi + de * uxi};
+        const float feta[4] = {br * dueta + r1 * ueta, bmx * dueta + x1 * ueta + gC * dp,
+                               bmy * dueta + y1 * ueta + gD * dp, bh * dueta + de * ueta};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           Fxi[v][r] = real ? fxi[v] : 0.0f;
@@ -1562,6 +1600,42 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       R[v] = mfma4(al, fw[v], R[v]);
   }
 
+  // The volume term of the cell average's flux, FP64: F_xi(avg) sum_q W_xi[q][m]
+  // + F_eta(avg) sum_q W_eta[q][m] for this lane's modes.
+  double vmean[4][KS];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int r = 0; r < KS; ++r)
+      vmean[v][r] = 0.0;
+  if constexpr (P > 0) {
+    double m[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      m[v] = __shfl(c64[v][0], j, 64) * kPhi0;
+    const double dA = ld_f64(rgeo, eoff, 0u), dB = ld_f64(rgeo, eoff, Eu * 8u),
+                 dC = ld_f64(rgeo, eoff, 2u * Eu * 8u), dD = ld_f64(rgeo, eoff, 3u * Eu * 8u);
+    const double *cs = mesh.table + Tri<P>::kCs;
+    double csx[KS], csy[KS];
+#pragma unroll
+    for (int r = 0; r < KS; ++r) {
+      const int mm = KS * grp + r < Np ? KS * grp + r : 0;
+      csx[r] = cs[mm];
+      csy[r] = cs[Np + mm];
+    }
+    const double inv = rcp(m[0]);
+    const double u = m[1] * inv, w = m[2] * inv;
+    const double pb = (mesh.gamma - 1.0) * (m[3] - 0.5 * (m[1] * u + m[2] * w));
+    const double uxi = dA * u + dB * w, ueta = dC * u + dD * w, h = m[3] + pb;
+    const double fxi[4] = {m[0] * uxi, m[1] * uxi + dA * pb, m[2] * uxi + dB * pb, h * uxi};
+    const double feta[4] = {m[0] * ueta, m[1] * ueta + dC * pb, m[2] * ueta + dD * pb, h * ueta};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < KS; ++r)
+        vmean[v][r] = fxi[v] * csx[r] + feta[v] * csy[r];
+  }
+
   // ---- finalise: lane holds R[v][row 4 grp + r] of element j, mode KS grp + r
   // for r < KS; FP64 from here
   float yf[4][KS];
@@ -1575,7 +1649,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       const int m = KS * grp + r;
       yf[v][r] = 0.0f;
       if (m < Np) {
-        const double res = static_cast<double>(R[v][r]) * invdet;
+        const double res = (static_cast<double>(R[v][r]) + vmean[v][r]) * invdet;
         double x;
         if (a.residual_only) {
           x = res;

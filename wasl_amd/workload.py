@@ -228,3 +228,48 @@ def run_served(reqs: list[Request], meshes: MeshCache, arrivals=None, precision=
 
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
+
+
+def _jacobi(n: int, alpha: float, x: np.ndarray) -> np.ndarray:
+    """P_n^(alpha, 0)(x) by the three-term recurrence."""
+    p0 = np.ones_like(x)
+    if n == 0:
+        return p0
+    p1 = 0.5 * ((alpha + 2.0) * x + alpha)
+    for k in range(2, n + 1):
+        s = 2 * k + alpha
+        a1 = 2.0 * k * (k + alpha) * (s - 2.0)
+        a2 = (s - 1.0) * alpha * alpha
+        a3 = (s - 2.0) * (s - 1.0) * s
+        a4 = 2.0 * (k + alpha - 1.0) * (k - 1.0) * s
+        p0, p1 = p1, ((a2 + a3 * x) * p1 - a4 * p0) / a1
+    return p1
+
+
+def dubiner_lattice(order: int, n: int = 0) -> np.ndarray:
+    """Orthonormal Dubiner basis [points][modes] (modes by degree, then i) at
+    the lattice points (xi, eta) = (-1 + 2a/n, -1 + 2b/n), a + b <= n, of the
+    reference triangle (n = 2 order + 2 by default): vertices, edges and
+    interior, so a maximum over these points samples a field's L-inf norm."""
+    n = n or 2 * order + 2
+    pts = [(-1.0 + 2.0 * a / n, -1.0 + 2.0 * b / n) for b in range(n + 1) for a in range(n + 1 - b)]
+    xi = np.array([p[0] for p in pts])
+    eta = np.array([p[1] for p in pts])
+    top = eta >= 1.0
+    a = np.where(top, -1.0, 2.0 * (1.0 + xi) / np.where(top, 1.0, 1.0 - eta) - 1.0)
+    cols = []
+    for deg in range(order + 1):
+        for i in range(deg + 1):
+            j = deg - i
+            norm = math.sqrt((2 * i + 1) * (deg + 1) / 2.0)
+            cols.append(norm * _jacobi(i, 0.0, a) * (0.5 * (1.0 - eta)) ** i *
+                        _jacobi(j, 2.0 * i + 1.0, eta))
+    return np.stack(cols, axis=1)
+
+
+def density_field_linf(a: np.ndarray, b: np.ndarray, order: int, elements: int) -> float:
+    """max |rho_a(x) - rho_b(x)| over the dubiner_lattice points of every
+    element, for two 2D Euler states in the U[variable][mode][element] layout."""
+    Np = (order + 1) * (order + 2) // 2
+    d = a.reshape(4, Np, elements)[0] - b.reshape(4, Np, elements)[0]
+    return float(np.max(np.abs(dubiner_lattice(order) @ d)))
