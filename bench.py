@@ -38,9 +38,9 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/advect_stage_524k.md and profiles/euler_line_stage_1m.md.
 # Other workloads report null.
 MEASURED_TRAFFIC = {
-    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7895.22 * 2.0 + 6149.39) * 1024.0,
+    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7893.31 * 2.0 + 6144.29) * 1024.0,
     ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7965.57 * 2.0 + 6146.72) * 1024.0,
-    ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (226335.0 * 2.0 + 262171.0) * 1024.0,
+    ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (226612.4 * 2.0 + 262157.8) * 1024.0,
     ("whip::advect_stage<1>", 1, 524288, "fp64"): (26979.5 * 2.0 + 12288.1) * 1024.0,
     ("whip::advect_stage<3>", 3, 524288, "fp64"): (51864.7 * 2.0 + 40960.1) * 1024.0,
     ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (96774.8 * 2.0 + 98346.8) * 1024.0,
@@ -167,8 +167,9 @@ def precision_error(mesh, p: int, precision: int, steps: int, end_time: float = 
     step of the initial state), or, with end_time > 0, the device loop of
     each engine to end_time, each choosing its own CFL dt and landing on
     end_time exactly. Density is compared as cell averages (L1 weighted by
-    element area) and as all modal coefficients (L-inf), relative to the
-    initial density's maximum cell average."""
+    element area), as all modal coefficients (L-inf, also relative to the
+    initial density's maximum cell average) and as the density field sampled
+    at lattice points of every element (L-inf)."""
     import wasl_amd
 
     runs = {}
@@ -185,9 +186,11 @@ def precision_error(mesh, p: int, precision: int, steps: int, end_time: float = 
             for _ in range(steps):
                 g.step(dt)
             t, n = steps * dt, steps
-        runs[name] = (g.state.reshape(4, -1, g.E)[0], t, n, time.perf_counter() - t0)
+        runs[name] = (g.state.reshape(4, -1, g.E)[0], t, n, time.perf_counter() - t0, g.state)
         E = g.E
         del g
+    from wasl_amd import workload as WL
+
     d = runs["reduced"][0] - runs["fp64"][0]
     avg = d[0] / np.sqrt(2.0)
     scale = float(np.max(np.abs(rho0)))
@@ -199,6 +202,7 @@ def precision_error(mesh, p: int, precision: int, steps: int, end_time: float = 
         "density_l1": float(np.sum(np.abs(avg) * detj) / np.sum(detj)),
         "density_linf": float(np.max(np.abs(d))),
         "density_linf_relative": float(np.max(np.abs(d))) / scale,
+        "density_field_linf": WL.density_field_linf(runs["reduced"][4], runs["fp64"][4], p, E),
     }
     if end_time > 0:
         out["reference"] = (f"FP64 engine, same initial state, each engine's own device dt to "
@@ -686,6 +690,9 @@ def main(argv=None) -> int:
     tile = (f"<{p}>" if args.precision != "fp64" else
             f"<{p},{eng.lanes}>" if eng.stage_kernel == 1 else f"<{p},{eng.lanes // 4}>")
     traffic = MEASURED_TRAFFIC.get((kernel + tile, p, E, args.precision))
+    ceilings = wasl_amd.fp64_ceilings()
+    log(f"FP64 ceilings on this device: vector ALU {ceilings[0]:.1f} TFLOP/s, matrix core "
+        f"{ceilings[1]:.1f} TFLOP/s")
 
     result = {
         "metric": METRIC,
@@ -730,6 +737,11 @@ def main(argv=None) -> int:
             "algorithmic_bytes_per_launch": bytes_stage,
             "fp64_tflops": flops_stage / (stage_ms * 1e-3) / 1e12,
             "fp64_frac": flops_stage / (stage_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            # what this device sustains on plain FMA chains: the FP64 matrix
+            # core (16x16x4) runs below the vector ALU, so the FP64 stage
+            # keeps its contractions on the vector ALU
+            "fp64_valu_ceiling_tflops": ceilings[0],
+            "fp64_mfma_ceiling_tflops": ceilings[1],
         },
         "cpu_baseline": None,
     }

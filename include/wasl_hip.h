@@ -285,6 +285,12 @@ int wasl_hip_capture_graphs(wasl_hip_engine *engine);
  * afterwards. */
 int wasl_hip_time_stages(wasl_hip_engine *engine, int reps, double dt, double *stage_us);
 
+/* Not part of the reference interface (measurement only): the FP64 rates in
+ * TFLOP/s that the current device sustains on its vector ALU (v_fma_f64
+ * chains) and on its matrix cores (v_mfma_f64_16x16x4f64 chains), each the
+ * best of three timed launches; the benchmark's roofline reports them. */
+int wasl_hip_fp64_ceilings(double *valu_tflops, double *mfma_tflops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,5 +11,5 @@ from . import meshgen, serve  # noqa: F401
 from .engine import (  # noqa: F401
     BF16, EULER, FP64, IC_EULER_REGIONS, IC_RAREFACTION, IC_SCALAR_CIRCLES, IC_SCALAR_PULSE,
     IC_SCALAR_REGIONS, IC_SCALAR_WAVE, IC_SMOOTH, IC_VORTEX, INFLOW, MIXED, OUTFLOW, PERIODIC,
-    SCALAR_ADVECTION, Engine, Mesh, SimulationError, WaslHipError, library,
+    SCALAR_ADVECTION, Engine, Mesh, SimulationError, WaslHipError, fp64_ceilings, library,
 )

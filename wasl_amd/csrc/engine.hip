@@ -2160,6 +2160,56 @@ int wasl_hip_capture_graphs(wasl_hip_engine *g) {
   });
 }
 
+// Not part of the reference interface: the FP64 rates the current device
+// sustains on its vector ALU (v_fma_f64) and on its matrix cores
+// (v_mfma_f64_16x16x4f64), in TFLOP/s, each the best of three timed launches
+// of 4096 blocks of 256 threads (16 waves per SIMD).
+int wasl_hip_fp64_ceilings(double *valu_tflops, double *mfma_tflops) {
+  return guarded([&] {
+    if (!valu_tflops || !mfma_tflops)
+      throw std::runtime_error("engine: null argument");
+    constexpr int kBlocks = 4096, kValuIters = 2048, kMfmaIters = 256;
+    hipStream_t stream = nullptr;
+    HIPCHECK(hipStreamCreate(&stream));
+    double *out = nullptr;
+    HIPCHECK(hipMalloc(&out, sizeof(double) * kBlocks * 256));
+    hipEvent_t t0, t1;
+    HIPCHECK(hipEventCreate(&t0));
+    HIPCHECK(hipEventCreate(&t1));
+    const auto best_ms = [&](auto launch) {
+      float best = 0.0f;
+      launch();   // warm
+      for (int r = 0; r < 3; ++r) {
+        HIPCHECK(hipEventRecord(t0, stream));
+        launch();
+        HIPCHECK(hipEventRecord(t1, stream));
+        HIPCHECK(hipEventSynchronize(t1));
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+        if (r == 0 || ms < best)
+          best = ms;
+      }
+      return static_cast<double>(best);
+    };
+    const double valu_ms = best_ms([&] {
+      hipLaunchKernelGGL(whip::fp64_valu_ceiling, dim3(kBlocks), dim3(256), 0, stream, out,
+                         kValuIters);
+    });
+    const double mfma_ms = best_ms([&] {
+      hipLaunchKernelGGL(whip::fp64_mfma_ceiling, dim3(kBlocks), dim3(256), 0, stream, out,
+                         kMfmaIters);
+    });
+    HIPCHECK(hipGetLastError());
+    // flops: lanes x iterations x 8 FMAs x 2; waves x iterations x 4 products x 2048
+    *valu_tflops = 2.0 * 8.0 * kValuIters * kBlocks * 256.0 / (valu_ms * 1e-3) / 1e12;
+    *mfma_tflops = 2048.0 * 4.0 * kMfmaIters * kBlocks * 4.0 / (mfma_ms * 1e-3) / 1e12;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    HIPCHECK(hipFree(out));
+    HIPCHECK(hipStreamDestroy(stream));
+  });
+}
+
 // Not part of the reference interface: switches per-stage HIP event timing
 // for wasl_hip_advance (used by the benchmark to time the stage kernel).
 int wasl_hip_set_timing(wasl_hip_engine *g, int on) {

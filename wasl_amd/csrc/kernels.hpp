@@ -2931,4 +2931,43 @@ __global__ __launch_bounds__(64) void scatter_limited_batch(const BatchLimit *ca
                   t.y * 64 + threadIdx.x, batch_out_traces(c, b.stage), mxtab);
 }
 
+
+// ------------------------------------------------- FP64 issue ceilings ----
+//
+// The two FP64 units a stage kernel could use, measured on the device: eight
+// independent v_fma_f64 chains per lane, and four independent
+// v_mfma_f64_16x16x4f64 chains per wave (one 16x16x4 product is 2048 flops).
+// Every lane writes one value, so neither loop is dead code.
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void fp64_valu_ceiling(double *out, int n) {
+  double x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    x[k] = 1.0 + 1e-3 * static_cast<double>(threadIdx.x + k);
+  const double a = 0.999999, b = 1e-9;
+  for (int i = 0; i < n; ++i)
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      x[k] = fma(x[k], a, b);
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    s += x[k];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void fp64_mfma_ceiling(double *out, int n) {
+  const int l = threadIdx.x & 63;
+  const double a = 1.0 + 1e-3 * l, b = 1.0 - 1e-3 * l;
+  mf_d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, c2 = c0, c3 = c0;
+  for (int i = 0; i < n; ++i) {
+    c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c3, 0, 0, 0);
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = c0[0] + c1[1] + c2[2] + c3[3];
+}
+
 }  // namespace whip

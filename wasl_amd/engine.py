@@ -107,6 +107,7 @@ def library() -> C.CDLL:
         "wasl_hip_set_timing": [_P, C.c_int],
         "wasl_hip_capture_graphs": [_P],
         "wasl_hip_time_stages": [_P, C.c_int, _d, _pd],
+        "wasl_hip_fp64_ceilings": [C.POINTER(_d), C.POINTER(_d)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -255,6 +256,14 @@ class Mesh:
         out = np.zeros(n)
         _check(library().wasl_hip_mesh_project(self.h, order, gamma, ic, prm, count, out))
         return out
+
+
+def fp64_ceilings() -> tuple[float, float]:
+    """(vector ALU, matrix core) FP64 TFLOP/s that the current device sustains
+    (wasl_hip_fp64_ceilings)."""
+    valu, mfma = C.c_double(0.0), C.c_double(0.0)
+    _check(library().wasl_hip_fp64_ceilings(C.byref(valu), C.byref(mfma)))
+    return valu.value, mfma.value
 
 
 class Engine:
