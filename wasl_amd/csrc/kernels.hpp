@@ -410,6 +410,13 @@ __device__ __forceinline__ T at_off(const T *base, unsigned byte_off) {
   return *(G *)(reinterpret_cast<const char *>(base) + byte_off);
 }
 
+// Element idx of an array below 4 GiB: one 32-bit offset per lane from a
+// scalar base, as at_off.
+template <typename T>
+__device__ __forceinline__ T at_idx(const T *base, unsigned idx) {
+  return at_off(base, idx * static_cast<unsigned>(sizeof(T)));
+}
+
 // Row r of an SoA array of E columns at column i (r wave-uniform).
 template <typename T>
 __device__ __forceinline__ T at_row(const T *base, int E, int r, int i) {
@@ -626,7 +633,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
           if constexpr (kPrefetch)
             x = nc[v][m];
           else
-            x = a.uin[(v * Np + m) * E + src];
+            x = at_row(a.uin, E, v * Np + m, src);
 #pragma unroll
           for (int q = 0; q < NE; ++q)
             so[q][v] += x * oth[q * Np + m];
@@ -647,7 +654,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
             double x = 0.0;
 #pragma unroll
             for (int m = 0; m < Np; ++m)
-              x += a.uin[(v * Np + m) * E + d] * db[m];
+              x += at_row(a.uin, E, v * Np + m, d) * db[m];
             t[v] = x;
           }
           so4 = {t[0], t[1], t[2], t[3]};
@@ -857,7 +864,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
   double c[Np], acc[Np];
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
-    c[m] = a.uin[(v * Np + m) * E + e];
+    c[m] = at_idx(a.uin, static_cast<unsigned>((v * Np + m) * E + e));
     acc[m] = 0.0;
   }
   const double gA = mesh.geo[e], gB = mesh.geo[E + e], gC = mesh.geo[2 * E + e],
@@ -1003,7 +1010,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
           double sd = 0.0;
 #pragma unroll
           for (int m = 0; m < Np; ++m)
-            sd += a.uin[(v * Np + m) * E + d] * db[m];
+            sd += at_idx(a.uin, static_cast<unsigned>((v * Np + m) * E + d)) * db[m];
           so = sd;
         } else {
           so = si;  // transmissive: exterior = own trace
@@ -1015,7 +1022,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
     if (next >= 0) {
 #pragma unroll
       for (int m = 0; m < Np; ++m)
-        nv[m] = a.uin[(v * Np + m) * E + next];
+        nv[m] = at_idx(a.uin, static_cast<unsigned>((v * Np + m) * E + next));
     }
     lane_exchange_fence();
     if (v < NE) {
