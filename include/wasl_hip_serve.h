@@ -59,7 +59,7 @@ typedef struct {
    * leave the tile to the autotuner (limited cases keep the autotuned tile, so
    * that their troubled-cell decisions are those of a single engine). A served class runs one launch over all its cases, so the
    * tile that suits a large mesh can beat the one a small case's autotuner
-   * picks. {0, 0}: the engine's autotuned tile. Defaults: P0 {0, 0}, P1 {1, 1},
+   * picks. {0, 0}: the engine's autotuned tile. Defaults: P0 {0, 0}, P1 {1, 2},
    * P2 {1, 2}, P3 {2, 4}. */
   int32_t fp64_tile[4][2];
 } wasl_hip_server_config;

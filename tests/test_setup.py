@@ -170,7 +170,7 @@ def test_server_config_defaults(wasl):
     variable split 4 lanes) and the struct layout the Python mirror assumes."""
     from wasl_amd.serve import ServerConfig, default_fp64_tiles
 
-    assert default_fp64_tiles() == {0: (0, 0), 1: (1, 1), 2: (1, 2), 3: (2, 4)}
+    assert default_fp64_tiles() == {0: (0, 0), 1: (1, 2), 2: (1, 2), 3: (2, 4)}
     c = ServerConfig()
     wasl.library().wasl_hip_server_config_default(ctypes.byref(c))
     assert (c.max_resident, c.steps_per_batch, c.setup_threads) == (256, 16, 4)

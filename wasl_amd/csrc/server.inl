@@ -588,9 +588,11 @@ void wasl_hip_server_config_default(wasl_hip_server_config *c) {
   c->setup_threads = 4;
   c->fp64_max_order = 1;
   c->fp64_limited = 1;
-  // fixed tiles beat each case's autotuned one on the 64-request serving mix
-  // (P1 point split 1 lane, P2 point split 2 lanes, P3 variable split 4 lanes)
-  const int32_t tiles[4][2] = {{0, 0}, {1, 1}, {1, 2}, {2, 4}};
+  // fixed tiles beat each case's autotuned one on the serving mix (P1 point
+  // split 2 lanes, P2 point split 2 lanes, P3 variable split 4 lanes); at P1
+  // two lanes (166 registers, three waves per SIMD) beat one (240, two) on
+  // the 1024-request mix by about 2 %
+  const int32_t tiles[4][2] = {{0, 0}, {1, 2}, {1, 2}, {2, 4}};
   for (int p = 0; p < 4; ++p) {
     c->fp64_tile[p][0] = tiles[p][0];
     c->fp64_tile[p][1] = tiles[p][1];
