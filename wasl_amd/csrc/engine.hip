@@ -567,10 +567,17 @@ template <int P> std::vector<float> mixed_tables() {
       std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 0) * 64 + l) * 2], hx, sizeof hx);
       std::memcpy(&t[X::kBgrad16 + ((tt * 2 + 1) * 64 + l) * 2], he, sizeof he);
     }
+    // Trace rows and lifting steps follow the face pairs: row 4g + t of the
+    // trace products and K step t of the lifting (row group g) are pair
+    // PPG g + t, that is edge (PPG g + t) / NE, point (PPG g + t) % NE.
+    const auto pair_basis = [&](int g, int tt, int m) {
+      const int q = X::PPG * g + tt;
+      return tt < X::PPG && q < X::PAIRS ? eb(q / NE, q % NE, m) : 0.0;
+    };
     for (int s = 0; s < KS; ++s)
-      t[X::kAface + s * 64 + l] = static_cast<float>(eb(i >> 2, i & 3, KS * kk + s));
-    for (int s = 0; s < NE; ++s)
-      t[X::kAlift + s * 64 + l] = static_cast<float>(eb(kk, s, X::row_mode(i)));
+      t[X::kAface + s * 64 + l] = static_cast<float>(pair_basis(i >> 2, i & 3, KS * kk + s));
+    for (int s = 0; s < X::PPG; ++s)
+      t[X::kAlift + s * 64 + l] = static_cast<float>(pair_basis(kk, s, X::row_mode(i)));
   }
   for (int k = 0; k < 3; ++k)
     for (int s = 0; s < NE; ++s) {

@@ -1197,12 +1197,19 @@ template <int P> struct Mx {
   static constexpr int row_mode(int row) {
     return (row & 3) < KS && KS * (row >> 2) + (row & 3) < Np ? KS * (row >> 2) + (row & 3) : Np;
   }
+  // Face points as (edge, point) pairs q = NE k + s; row group g serves
+  // pairs PPG g .. PPG g + PPG - 1 (at P3 the twelve pairs fill all four row
+  // groups; below P3 the last row group has none). Row 4g + t of the trace
+  // products is pair PPG g + t; the pairs of a row group span at most two
+  // edges.
+  static constexpr int PAIRS = 3 * NE;
+  static constexpr int PPG = (PAIRS + 3) / 4;
   // per-lane operand tables, floats, each [..][64]
   static constexpr int kAvol = 0;                         // [MT][KS]
   static constexpr int kAgrad = kAvol + MT * KS * 64;     // [MT][4 r][2 d]
   static constexpr int kAface = kAgrad + MT * 8 * 64;     // [KS]
-  static constexpr int kAlift = kAface + KS * 64;         // [NE]
-  static constexpr int kBe = kAlift + NE * 64;            // [3][NE][Np] own-edge basis
+  static constexpr int kAlift = kAface + KS * 64;         // [PPG]
+  static constexpr int kBe = kAlift + PPG * 64;           // [3][NE][Np] own-edge basis
   static constexpr int kWe = kBe + 3 * NE * Np;           // [3][NE] edge weights
   // bf16 operands (pairs of bf16 per float slot), each lane 4 bf16 = 2 slots
   static constexpr int kBvol16 = (kWe + 3 * NE + 3) / 4 * 4;   // [MT][64][2], 16-byte aligned
@@ -1217,8 +1224,9 @@ typedef float mx_f4 __attribute__((ext_vector_type(4)));
 // waves): every global read sits in registers from the start of the kernel.
 // The deviation-form volume holds the cell average's state through the
 // volume products; P1 fits 128 registers (four waves) and P2 and P3 168
-// (three waves). Measured on MI355X at P2: three waves without spills beat
-// four with a 20-byte spill (155.9 against 157.5 us at 524288 triangles).
+// (three waves). Measured on MI355X at P2: three and four waves run alike
+// (146.3 against 146.1 us at 524288 triangles), and the batched kernel
+// spills at four.
 template <int P, bool BF16> constexpr int mx_waves() {
   return P == 0 ? 5 : P == 1 ? 4 : 3;
 }
@@ -1310,6 +1318,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
                                          double *red) {
   using X = Mx<P>;
   constexpr int Np = X::Np, NQ = X::NQ, NE = X::NE, MT = X::MT, KS = X::KS;
+  constexpr int PPG = X::PPG, PAIRS = X::PAIRS;
   constexpr int kStaged = mx_staged<P, BF16>();
 
   const int E = mesh.E;
@@ -1358,20 +1367,28 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
               gB = static_cast<float>(ld_f64(rgeo, eoff, Eu * 8u)),
               gC = static_cast<float>(ld_f64(rgeo, eoff, 2u * Eu * 8u)),
               gD = static_cast<float>(ld_f64(rgeo, eoff, 3u * Eu * 8u));
-  const int k = grp < 3 ? grp : 0;   // row group 3 idles on edge 0
-  const unsigned koff = static_cast<unsigned>(k) * Eu + static_cast<unsigned>(e);
-  const int meta = ld_i32(rsrc(mesh.meta, static_cast<size_t>(3) * Eu * 4), koff * 4u, 0u);
-  const int nbr = ld_i32(rsrc(mesh.nbr, static_cast<size_t>(3) * Eu * 4), koff * 4u, 0u);
+  // The records of the (at most two) edges of this row group's face pairs;
+  // a row group without pairs idles on pair 0.
+  const int pfirst = PPG * grp < PAIRS ? PPG * grp : 0;
+  const int plast = PPG * grp + PPG - 1 < PAIRS ? PPG * grp + PPG - 1 : pfirst;
+  const int kedge[2] = {pfirst / NE, plast / NE};
+  const Rsrc rmeta = rsrc(mesh.meta, static_cast<size_t>(3) * Eu * 4);
+  const Rsrc rnbr = rsrc(mesh.nbr, static_cast<size_t>(3) * Eu * 4);
   const Rsrc rfn = rsrc(mesh.fn, static_cast<size_t>(9) * Eu * 8);
-  const unsigned foff = (static_cast<unsigned>(3 * k) * Eu + static_cast<unsigned>(e)) * 8u;
-  const float nx = static_cast<float>(ld_f64(rfn, foff, 0u));
-  const float ny = static_cast<float>(ld_f64(rfn, foff, Eu * 8u));
-  const float scale = static_cast<float>(ld_f64(rfn, foff, 2u * Eu * 8u));
-  const bool boundary = meta & kMetaBoundary;
-  const bool is_left = meta & kMetaLeft;
-  const bool rev = meta & kMetaReverse;
-  const int kl = meta_kl(meta);
-  const int kn = is_left ? meta_kr(meta) : kl;   // the neighbour's local edge
+  // Up to P2 a row group's pairs are exactly one edge's points.
+  constexpr bool kOneEdge = PPG == NE;
+  int meta2[2], nbr2[2];
+  float nx2[2], ny2[2], scale2[2];
+#pragma unroll
+  for (int h = 0; h < (kOneEdge ? 1 : 2); ++h) {
+    const unsigned koff = static_cast<unsigned>(kedge[h]) * Eu + static_cast<unsigned>(e);
+    meta2[h] = ld_i32(rmeta, koff * 4u, 0u);
+    nbr2[h] = ld_i32(rnbr, koff * 4u, 0u);
+    const unsigned foff = (static_cast<unsigned>(3 * kedge[h]) * Eu + static_cast<unsigned>(e)) * 8u;
+    nx2[h] = static_cast<float>(ld_f64(rfn, foff, 0u));
+    ny2[h] = static_cast<float>(ld_f64(rfn, foff, Eu * 8u));
+    scale2[h] = static_cast<float>(ld_f64(rfn, foff, 2u * Eu * 8u));
+  }
 
   RunCtl *ctl = a.ctl;
   double dt = a.dt;
@@ -1388,18 +1405,26 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   }
   __syncthreads();
 
-  // The neighbour's trace on edge `grp` (64 contiguous bytes; its index
-  // arrived with the first round trip): requested now, its latency
-  // overlapping the volume products.
-  float4 ntr[4];
+  // The neighbour's trace at each of this row group's face pairs (16 bytes
+  // each; the indices arrived with the first round trip): requested now, their
+  // latency overlapping the volume products.
+  float4 ntr[PPG];
   {
     const Rsrc rt = rsrc(a.tin, static_cast<size_t>(Eu) * 3 * 64);
-    const unsigned toff =
-        (static_cast<unsigned>(boundary ? e : nbr) * 3u + static_cast<unsigned>(boundary ? 0 : kn)) *
-        64u;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      ntr[s] = ld_f32x4(rt, toff, 16u * s);
+    for (int t = 0; t < PPG; ++t) {
+      const int q = PPG * grp + t < PAIRS ? PPG * grp + t : pfirst;
+      const int h = kOneEdge || q / NE == kedge[0] ? 0 : 1;
+      const int mt = meta2[h], sp = kOneEdge ? t : q - NE * (q / NE);
+      const bool bnd = mt & kMetaBoundary, lft = mt & kMetaLeft;
+      const int kn = lft ? meta_kr(mt) : meta_kl(mt);   // the neighbour's local edge
+      const int sn = (mt & kMetaReverse) ? NE - 1 - sp : sp;
+      const unsigned toff =
+          (static_cast<unsigned>(bnd ? e : nbr2[h]) * 3u + static_cast<unsigned>(bnd ? 0 : kn)) *
+              64u +
+          16u * static_cast<unsigned>(bnd ? 0 : sn);
+      ntr[t] = ld_f32x4(rt, toff, 0u);
+    }
   }
 
   mx_f4 R[4];
@@ -1555,16 +1580,22 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
   }
 
 #pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    const F4 own = {T[0][s], T[1][s], T[2][s], T[3][s]};
+  for (int t = 0; t < PPG; ++t) {
+    const bool pair = PPG * grp + t < PAIRS;
+    const int q = pair ? PPG * grp + t : pfirst;
+    const int h = kOneEdge || q / NE == kedge[0] ? 0 : 1;
+    const int meta = meta2[h], sp = kOneEdge ? t : q - NE * (q / NE);
+    const bool boundary = meta & kMetaBoundary;
+    const bool is_left = meta & kMetaLeft;
+    const int kl = meta_kl(meta);
+    const F4 own = {T[0][t], T[1][t], T[2][t], T[3][t]};
     F4 oth = own;   // transmissive: exterior = own trace
     if (!boundary) {
-      const float4 n4 = ntr[rev ? NE - 1 - s : s];
-      oth = {n4.x, n4.y, n4.z, n4.w};
+      oth = {ntr[t].x, ntr[t].y, ntr[t].z, ntr[t].w};
     } else if (mesh.bdonor) {
       const int slot = meta_slot(meta);
-      const int d = mesh.bdonor[slot * NE + s];
-      const double *db = mesh.bbasis + (slot * NE + s) * Np;
+      const int d = mesh.bdonor[slot * NE + sp];
+      const double *db = mesh.bbasis + (slot * NE + sp) * Np;
       const unsigned doff = static_cast<unsigned>(d) * 8u;
       // The donor basis and a variable's modes (half of them from P3) are
       // requested together, so a wave waits once per group, not per load.
@@ -1592,16 +1623,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       }
       oth = {x[0], x[1], x[2], x[3]};
     }
-    const F4 f = hllc_f(g, is_left ? own : oth, is_left ? oth : own, nx, ny);
-    const float wq = tab[X::kWe + kl * NE + s] * scale;
+    const F4 f = hllc_f(g, is_left ? own : oth, is_left ? oth : own, nx2[h], ny2[h]);
+    const float wq = tab[X::kWe + kl * NE + sp] * scale2[h];
     const float sw = is_left ? -wq : wq;
-    // Row group 3 has no edge: its (meaningless) flux is selected away, not
-    // multiplied by zero, since a NaN would survive the product and spread
-    // through the matrix core to every mode of the element.
-    const bool edge = grp < 3;
-    const float fw[4] = {edge ? f.r * sw : 0.0f, edge ? f.mx * sw : 0.0f, edge ? f.my * sw : 0.0f,
-                         edge ? f.E * sw : 0.0f};
-    const float al = tab[X::kAlift + s * 64 + lane];
+    // A row group without this pair: its (meaningless) flux is selected away,
+    // not multiplied by zero, since a NaN would survive the product and
+    // spread through the matrix core to every mode of the element.
+    const float fw[4] = {pair ? f.r * sw : 0.0f, pair ? f.mx * sw : 0.0f,
+                         pair ? f.my * sw : 0.0f, pair ? f.E * sw : 0.0f};
+    const float al = tab[X::kAlift + t * 64 + lane];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       R[v] = mfma4(al, fw[v], R[v]);
@@ -1693,12 +1723,17 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       for (int v = 0; v < 4; ++v)
         Tn[v] = mfma4(af, yf[v][s], Tn[v]);
     }
-    if (valid && grp < 3) {
+    if (valid) {
       const Rsrc rt = rsrc(a.tout, static_cast<size_t>(Eu) * 3 * 64);
-      const unsigned toff = (static_cast<unsigned>(e) * 3u + static_cast<unsigned>(grp)) * 64u;
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
-        st_f32x4(make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]), rt, toff, 16u * s);
+      for (int t = 0; t < PPG; ++t) {
+        const int q = PPG * grp + t;   // pair: edge q / NE, point q % NE
+        if (q < PAIRS)
+          st_f32x4(make_float4(Tn[0][t], Tn[1][t], Tn[2][t], Tn[3][t]), rt,
+                   (static_cast<unsigned>(e) * 3u + static_cast<unsigned>(q / NE)) * 64u +
+                       16u * static_cast<unsigned>(q % NE),
+                   0u);
+      }
     }
   }
 
@@ -1777,11 +1812,14 @@ __device__ __forceinline__ void mx_traces_body(const double *U, int E, float *T,
       Tn[v] = mfma4(af, c, Tn[v]);
     }
   }
-  if (e_raw < E && grp < 3) {
-    float4 *dst = reinterpret_cast<float4 *>(T + (static_cast<size_t>(e) * 3 + grp) * 16);
+  if (e_raw < E) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      dst[s] = make_float4(Tn[0][s], Tn[1][s], Tn[2][s], Tn[3][s]);
+    for (int t = 0; t < X::PPG; ++t) {
+      const int q = X::PPG * grp + t;   // pair: edge q / NE, point q % NE
+      if (q < X::PAIRS)
+        reinterpret_cast<float4 *>(T + (static_cast<size_t>(e) * 3 + q / X::NE) * 16)[q % X::NE] =
+            make_float4(Tn[0][t], Tn[1][t], Tn[2][t], Tn[3][t]);
+    }
   }
 }
 
