@@ -2321,6 +2321,20 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
   const double g = mesh.gamma, g1 = g - 1.0;
   double c[4][Np];
   load_coefficients<Np>(a.uin, E, e, c);
+  // The neighbours' traces come from the adjacent lanes (the cells run left
+  // to right, one per lane). Only a wave's first lane (or the first cell) and
+  // its last lane (or the last cell) read a neighbour from memory, requested
+  // here with the own coefficients so that no second round trip follows.
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int en = e > 0 ? e - 1 : (mesh.periodic ? E - 1 : e);
+  const int ep = e < E - 1 ? e + 1 : (mesh.periodic ? 0 : e);
+  const bool far_l = lane == 0 || e == 0;
+  const bool far_r = lane == 63 || e_raw >= E - 1;
+  double cl[4][Np], cr[4][Np];
+  if (far_l)
+    load_coefficients<Np>(a.uin, E, en, cl);
+  if (far_r)
+    load_coefficients<Np>(a.uin, E, ep, cr);
   double dt = a.dt;
   if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
     return;
@@ -2329,11 +2343,6 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
   for (int i = threadIdx.x; i < D::kTable; i += blockDim.x)
     tab[i] = mesh.table[i];
   __syncthreads();
-  const int en = e > 0 ? e - 1 : (mesh.periodic ? E - 1 : e);
-  const int ep = e < E - 1 ? e + 1 : (mesh.periodic ? 0 : e);
-  double cl[4][Np], cr[4][Np];
-  load_coefficients<Np>(a.uin, E, en, cl);
-  load_coefficients<Np>(a.uin, E, ep, cr);
 
   double acc[4][Np];
 #pragma unroll
@@ -2353,10 +2362,18 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
   }
   const double *bl = tab + D::kL, *br = tab + D::kR;
   const C4 own_l = trace_of<Np>(c, bl), own_r = trace_of<Np>(c, br);
+  // the left neighbour's right trace and the right neighbour's left trace,
+  // the same bits the neighbour evaluates for its own faces
+  C4 ext_l = {__shfl_up(own_r.r, 1, 64), __shfl_up(own_r.mx, 1, 64),
+              __shfl_up(own_r.my, 1, 64), __shfl_up(own_r.E, 1, 64)};
+  C4 ext_r = {__shfl_down(own_l.r, 1, 64), __shfl_down(own_l.mx, 1, 64),
+              __shfl_down(own_l.my, 1, 64), __shfl_down(own_l.E, 1, 64)};
   // outflow ends: the exterior is the owner's own trace at that end
   const bool open_l = e == 0 && !mesh.periodic, open_r = e == E - 1 && !mesh.periodic;
-  const C4 ext_l = open_l ? own_l : trace_of<Np>(cl, br);
-  const C4 ext_r = open_r ? own_r : trace_of<Np>(cr, bl);
+  if (far_l)
+    ext_l = open_l ? own_l : trace_of<Np>(cl, br);
+  if (far_r)
+    ext_r = open_r ? own_r : trace_of<Np>(cr, bl);
   const C4 fl = hllc(g, ext_l, own_l, 1.0, 0.0);
   const C4 fr = hllc(g, own_r, ext_r, 1.0, 0.0);
   const double vl[4] = {fl.r, fl.mx, fl.my, fl.E}, vr[4] = {fr.r, fr.mx, fr.my, fr.E};
