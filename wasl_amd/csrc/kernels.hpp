@@ -1976,69 +1976,83 @@ __device__ __forceinline__ void line_body(const LineMeshDev mesh, const StageArg
                                           bool leader, double *tab) {
   using D = Line<P>;
   constexpr int Np = D::Np, NQ = D::NQ;
+  const int E = mesh.E;
+  const int e_raw = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+  const bool valid = e_raw < E;
+  const int e = valid ? e_raw : E - 1;
+  // Own coefficients first, ahead of the step-control read. The neighbours'
+  // traces come from the adjacent lanes (the cells run left to right, one
+  // per lane); only a wave's end lanes and the end cells read a neighbour
+  // from memory, requested here too.
+  double c[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m)
+    c[m] = a.uin[e * Np + m];
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const bool far_l = lane == 0 || e == 0;
+  const bool far_r = lane == 63 || e_raw >= E - 1;
+  double cl[Np], cr[Np];
+  if (far_l) {
+    const int en = e > 0 ? e - 1 : E - 1;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      cl[m] = a.uin[en * Np + m];
+  }
+  if (far_r) {
+    const int ep = e < E - 1 ? e + 1 : 0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      cr[m] = a.uin[ep * Np + m];
+  }
   double dt = a.dt;
   if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
     return;
   for (int i = threadIdx.x; i < D::kTable; i += blockDim.x)
     tab[i] = mesh.table[i];
   __syncthreads();
-  const int E = mesh.E;
-  const int e = block * blockDim.x + threadIdx.x;
-  if (e < E) {
-    const double sa = mesh.wavespeed;
-    double c[Np], acc[Np];
+  const double sa = mesh.wavespeed;
+  double acc[Np];
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      c[m] = a.uin[e * Np + m];
-      acc[m] = 0.0;
-    }
-    const double *bl = tab + D::kL, *br = tab + D::kR;
-    if constexpr (P > 0) {
+  for (int m = 0; m < Np; ++m)
+    acc[m] = 0.0;
+  const double *bl = tab + D::kL, *br = tab + D::kR;
+  if constexpr (P > 0) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        double u = 0.0;
-#pragma unroll
-        for (int m = 0; m < Np; ++m)
-          u += c[m] * tab[D::kB + q * Np + m];
-        const double f = sa * u;
-#pragma unroll
-        for (int m = 0; m < Np; ++m)
-          acc[m] += f * tab[D::kW + q * Np + m];
-      }
-    }
-    const auto trace = [&](int el, const double *b) {
-      double t = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+      double u = 0.0;
 #pragma unroll
       for (int m = 0; m < Np; ++m)
-        t += a.uin[el * Np + m] * b[m];
-      return t;
-    };
-    double own_l = 0.0, own_r = 0.0;
+        u += c[m] * tab[D::kB + q * Np + m];
+      const double f = sa * u;
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      own_l += c[m] * bl[m];
-      own_r += c[m] * br[m];
+      for (int m = 0; m < Np; ++m)
+        acc[m] += f * tab[D::kW + q * Np + m];
     }
-    // left face of e
-    double fl;
-    if (e == 0) {
-      const double ext = mesh.left_kind == 1   ? mesh.left_value
-                         : mesh.left_kind == 2 ? own_l
-                                               : trace(E - 1, br);
-      fl = upwind(sa, ext, own_l);
-    } else {
-      fl = upwind(sa, trace(e - 1, br), own_l);
-    }
-    // right face of e
-    double fr;
-    if (e == E - 1) {
-      const double ext = mesh.right_kind == 1   ? mesh.right_value
-                         : mesh.right_kind == 2 ? own_r
-                                                : trace(0, bl);
-      fr = upwind(sa, own_r, ext);
-    } else {
-      fr = upwind(sa, own_r, trace(e + 1, bl));
-    }
+  }
+  const auto trace = [&](const double (&x)[Np], const double *b) {
+    double t = 0.0;
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      t += x[m] * b[m];
+    return t;
+  };
+  const double own_l = trace(c, bl), own_r = trace(c, br);
+  // the left neighbour's right trace and the right neighbour's left trace:
+  // the same bits the neighbour evaluates for its own faces
+  double ext_l = __shfl_up(own_r, 1, 64), ext_r = __shfl_down(own_l, 1, 64);
+  if (far_l)
+    ext_l = e > 0 ? trace(cl, br)
+                  : mesh.left_kind == 1   ? mesh.left_value
+                  : mesh.left_kind == 2 ? own_l
+                                        : trace(cl, br);
+  if (far_r)
+    ext_r = e < E - 1 ? trace(cr, bl)
+                      : mesh.right_kind == 1   ? mesh.right_value
+                      : mesh.right_kind == 2 ? own_r
+                                             : trace(cr, bl);
+  const double fl = upwind(sa, ext_l, own_l);   // left face of e
+  const double fr = upwind(sa, own_r, ext_r);   // right face of e
+  if (valid) {
     const double ij = mesh.inv_jac[e];
 #pragma unroll
     for (int m = 0; m < Np; ++m) {
