@@ -43,7 +43,7 @@ MEASURED_TRAFFIC = {
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (226421.8 * 2.0 + 262158.0) * 1024.0,
     ("whip::advect_stage<1>", 1, 524288, "fp64"): (26979.5 * 2.0 + 12288.1) * 1024.0,
     ("whip::advect_stage<3>", 3, 524288, "fp64"): (51864.7 * 2.0 + 40960.1) * 1024.0,
-    ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (96774.8 * 2.0 + 98346.8) * 1024.0,
+    ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (92695.5 * 2.0 + 98315.6) * 1024.0,
 }
 
 
