@@ -475,13 +475,20 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     nb0 = mesh.nbr[k0 * E + e];
   }
 
-  // Up to P1 the registers also hold the metrics and 1/detJ from here on;
-  // at P2 and P3 the point split has no room for them before the volume.
+  // Up to P1 the registers also hold the metrics and 1/detJ from here on.
+  // At P2 with 4 lanes they hold the four metrics (the volume then starts
+  // without a memory round trip); with fewer lanes, and at P3, there is no
+  // room for them before the volume.
   constexpr bool kEarlyMetrics = P <= 1;
   double gm[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr bool kEarlyGeo = P == 2 && LANES == 4;
   if constexpr (kEarlyMetrics) {
 #pragma unroll
     for (int i = 0; i < 5; ++i)
+      gm[i] = at_row(mesh.geo, E, i, e);
+  } else if constexpr (kEarlyGeo) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
       gm[i] = at_row(mesh.geo, E, i, e);
   }
   // this thread's share of the reference tables, staged to LDS below
@@ -542,7 +549,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   // Volume: sum_q (W_xi[q][m] F_xi + W_eta[q][m] F_eta) with the reference
   // fluxes F_xi = A Fx + B Fy, F_eta = C Fx + D Fy (= detJ J^-1 F).
   if constexpr (P > 0) {
-    if constexpr (!kEarlyMetrics) {
+    if constexpr (!kEarlyMetrics && !kEarlyGeo) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         gm[i] = at_row(mesh.geo, E, i, e);
@@ -577,17 +584,26 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     }
   }
 
-  // U^n of the variables this lane finalises (stages 2 and 3): up to P1 it
-  // is requested here, its latency overlapping the faces; at P2 and P3 the
-  // registers go to the faces and it is read in the epilogue.
-  double unr[4][Np];
+  // Lane `sub` finalises variables sub*VPL .. sub*VPL + VPL - 1. Their U^n
+  // (stages 2 and 3): up to P1 it is requested here, its latency overlapping
+  // the faces; at P2 and P3 the registers go to the faces and it is read in
+  // the epilogue.
+  constexpr int VPL = 4 / LANES;
+  double uo[VPL][Np];
+#pragma unroll
+  for (int j = 0; j < VPL; ++j)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      uo[j][m] = 0.0;
   if constexpr (kEarlyMetrics) {
+    if (a.ca != 0.0 && !a.residual_only) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+      for (int j = 0; j < VPL; ++j)
 #pragma unroll
-      for (int m = 0; m < Np; ++m)
-        unr[v][m] =
-            (v * LANES / 4 == sub && a.ca != 0.0) ? at_row(a.un, E, v * Np + m, e) : 0.0;
+        for (int m = 0; m < Np; ++m)
+          uo[j][m] = at_idx(a.un, static_cast<unsigned>((sub * VPL + j) * Np + m) *
+                                      static_cast<unsigned>(E) + static_cast<unsigned>(e));
+    }
   }
 
   // 1/detJ is requested here (from P2 on), its latency overlapping the faces.
@@ -676,11 +692,51 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     }
   }
 
+  // Reduce-scatter over the element's lanes: each lane keeps the full sums of
+  // the variables it finalises. Partners one lane apart combine first, then
+  // two apart, so every sum is (x0 + x1) + (x2 + x3) on whichever lane holds
+  // it.
+  double res[VPL][Np];
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
+  for (int m = 0; m < Np; ++m) {
+    if constexpr (LANES == 1) {
 #pragma unroll
-    for (int m = 0; m < Np; ++m)
-      acc[v][m] = lane_sum<LANES>(acc[v][m]);
+      for (int v = 0; v < 4; ++v)
+        res[v][m] = acc[v][m];
+    } else if constexpr (LANES == 2) {
+      const bool odd = sub & 1;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double send = odd ? acc[k][m] : acc[2 + k][m];
+        const double keep = odd ? acc[2 + k][m] : acc[k][m];
+        res[k][m] = keep + dpp<kQuadXor1>(send);
+      }
+    } else {
+      const bool odd = sub & 1, hi = sub & 2;
+      double kp[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double send = odd ? acc[2 * k][m] : acc[2 * k + 1][m];
+        const double keep = odd ? acc[2 * k + 1][m] : acc[2 * k][m];
+        kp[k] = keep + dpp<kQuadXor1>(send);
+      }
+      const double send = hi ? kp[0] : kp[1];
+      const double keep = hi ? kp[1] : kp[0];
+      res[0][m] = keep + dpp<kQuadXor2>(send);
+    }
+  }
+  // this lane's share of the element's own coefficients
+  const auto own_c = [&](int j, int m) -> double {
+    if constexpr (LANES == 1) {
+      return c[j][m];
+    } else if constexpr (LANES == 2) {
+      return (sub & 1) ? c[2 + j][m] : c[j][m];
+    } else {
+      const double lo = (sub & 1) ? c[1][m] : c[0][m];
+      const double hi = (sub & 1) ? c[3][m] : c[2][m];
+      return (sub & 2) ? hi : lo;
+    }
+  };
 
   if (un_lds) {
     __builtin_amdgcn_s_waitcnt(0);   // this wave's slab has landed in LDS
@@ -688,57 +744,72 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   }
   const int el_w = static_cast<int>(threadIdx.x) / LANES - wv * EPW;   // element within the wave
 
-  // Finalise: lane `sub` owns variables [sub*4/LANES, (sub+1)*4/LANES).
-  double out0[4] = {0.0, 0.0, 0.0, 0.0};
-  double dens[Np];
+  // Finalise. Every U^n read comes before the first store: the slab and
+  // global reads would otherwise wait for the stores issued ahead of them.
+  // One loop per source: a single loop choosing between the slab and global
+  // memory compiles to flat loads, which wait for LDS and vector memory alike.
+  if constexpr (!kEarlyMetrics) {
+    if (a.ca != 0.0 && !a.residual_only) {
+      bool from_slab = false;
+      if constexpr (kUnLds)
+        from_slab = un_lds;
+      if (from_slab) {
 #pragma unroll
-  for (int m = 0; m < Np; ++m)
-    dens[m] = 0.0;
-  // One loop per source of U^n. A single loop choosing between the LDS slab
-  // and global memory compiles to flat loads, and every flat load waits for
-  // all outstanding vector-memory operations, the previous store included.
-  const auto finalize = [&](auto un_at) {
+        for (int j = 0; j < VPL; ++j)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      if (v * LANES / 4 != sub)
-        continue;
+          for (int m = 0; m < Np; ++m)
+            uo[j][m] = unl[wv][((sub * VPL + j) * Np + m) * EPW + el_w];
+      } else {
 #pragma unroll
-      for (int m = 0; m < Np; ++m) {
-        const double r = acc[v][m] * invdet;
-        const int idx = (v * Np + m) * E + e;
-        double y;
-        if (a.residual_only) {
-          y = r;
-        } else {
-          y = a.cb * (c[v][m] + dt * r);
-          if (a.ca != 0.0)
-            y += a.ca * un_at(v, m, idx);
-        }
-        if (valid)
-          a.out[idx] = y;
-        if (m == 0)
-          out0[v] = y;
-        if (v == 0)
-          dens[m] = y;
+        for (int j = 0; j < VPL; ++j)
+#pragma unroll
+          for (int m = 0; m < Np; ++m)
+            uo[j][m] = at_idx(a.un, static_cast<unsigned>((sub * VPL + j) * Np + m) *
+                                            static_cast<unsigned>(E) +
+                                        static_cast<unsigned>(e));
       }
     }
-  };
-  if constexpr (kEarlyMetrics) {
-    finalize([&](int v, int m, int) { return unr[v][m]; });
-  } else if constexpr (kUnLds) {
-    if (un_lds)
-      finalize([&](int v, int m, int) { return unl[wv][(v * Np + m) * EPW + el_w]; });
-    else
-      finalize([&](int, int, int idx) { return a.un[idx]; });
-  } else {
-    finalize([&](int, int, int idx) { return a.un[idx]; });
   }
-
-  // Gather the new cell averages to every lane of the element: only the
-  // owning lane holds a nonzero value, so the lane sum is exact.
+  double ys[VPL][Np];
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
-    out0[v] = lane_sum<LANES>(out0[v]);
+  for (int j = 0; j < VPL; ++j)
+#pragma unroll
+    for (int m = 0; m < Np; ++m) {
+      const int row = (sub * VPL + j) * Np + m;
+      const double r = res[j][m] * invdet;
+      double y;
+      if (a.residual_only) {
+        y = r;
+      } else {
+        y = a.cb * (own_c(j, m) + dt * r);
+        if (a.ca != 0.0)
+          y += a.ca * uo[j][m];
+      }
+      if (valid)
+        a.out[row * E + e] = y;
+      ys[j][m] = y;
+    }
+
+  // The new cell averages of all four variables on the element's first lane,
+  // and its density modes for the sensor (the first lane owns variable 0).
+  double out0[4];
+  if constexpr (LANES == 1) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      out0[v] = ys[v][0];
+  } else if constexpr (LANES == 2) {
+    constexpr int kPairLead = 0xF5;   // quad_perm [1,1,3,3]
+    out0[0] = ys[0][0];
+    out0[1] = ys[1][0];
+    out0[2] = dpp<kPairLead>(ys[0][0]);
+    out0[3] = dpp<kPairLead>(ys[1][0]);
+  } else {
+    out0[0] = ys[0][0];
+    out0[1] = dpp<quad_bcast<1>()>(ys[0][0]);
+    out0[2] = dpp<quad_bcast<2>()>(ys[0][0]);
+    out0[3] = dpp<quad_bcast<3>()>(ys[0][0]);
+  }
+  const double (&dens)[Np] = ys[0];
 
   if (a.sensor && !a.residual_only && sub == 0 && valid) {
     // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
