@@ -160,7 +160,10 @@ int wasl_hip_engine_create(const wasl_hip_mesh *mesh, const wasl_hip_numerics *n
 void wasl_hip_engine_destroy(wasl_hip_engine *engine);
 
 /* Sizes: 0 elements, 1 modes, 2 variables, 3 state doubles, 4 faces,
- * 5 device bytes held, 6 lanes per element in use. */
+ * 5 device bytes held, 6 lanes per element in use, 7 limiter check points,
+ * 8 stage kernel in use, 9 boundary faces kept as boundary (transmissive, or
+ * periodic with per-point donors), 10 conforming periodic face pairs recorded
+ * as interior faces. */
 int64_t wasl_hip_engine_size(const wasl_hip_engine *engine, int what);
 
 /* Periodic donors [F][p+1] and donor basis [F][p+1][Np]

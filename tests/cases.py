@@ -60,6 +60,21 @@ def vortex_mesh(n=12, seed=MG.DEFAULT_SEED):
     return MG.jittered_triangles(n, (0.0, 0.0), (10.0, 10.0), 0.15, seed)
 
 
+def nonconforming_vortex_mesh(n=10, seed=MG.DEFAULT_SEED):
+    """vortex_mesh(n) with the nodes of the x = 10 boundary (corners kept)
+    moved along it by a quarter of the spacing. The periodic images of the
+    x = 0 boundary's Gauss points then fall between the x = 10 boundary's
+    Gauss points (and the other way round), so those 2 n faces keep the
+    per-point donor lookup, while the n conforming y = 0 / y = 10 pairs are
+    recorded as interior faces."""
+    m = vortex_mesh(n, seed)
+    xy = m.node_xy.copy()
+    on = (xy[:, 0] == 10.0) & (xy[:, 1] > 0.0) & (xy[:, 1] < 10.0)
+    assert on.sum() == n - 1
+    xy[on, 1] += 0.25 * 10.0 / n
+    return MG.TriangleMeshInput(m.node_tags, xy, m.triangles, m.box)
+
+
 def rarefaction_mesh(n=10, seed=MG.DEFAULT_SEED):
     return MG.jittered_triangles(n, (-1.0, -1.0), (1.0, 1.0), 0.15, seed)
 

@@ -38,6 +38,19 @@ def test_residual_matches_oracle(gpu, p):
 
 
 @pytest.mark.parametrize("p", [0, 1, 2, 3])
+def test_residual_periodic_donors(gpu, p):
+    """Per-point periodic donors on the x boundaries (shifted x = 10 nodes)."""
+    m = K.nonconforming_vortex_mesh(9)
+    g, o = _engine(gpu, m, p), K.oracle_tri_scalar(m, p)
+    assert g.boundary_faces == 18
+    rng = np.random.default_rng(50 + p)
+    U = o.project(O.IC_SCALAR_WAVE, K.SCALAR_WAVE) + 1e-3 * rng.standard_normal(o.state_size)
+    err = K.rel_inf(g.residual(U), o.residual(U))
+    print(f"P{p} donor-path residual rel {err:.2e}", flush=True)
+    assert err <= K.STEP_TOL
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3])
 def test_reference_vectors(gpu, p):
     f = np.load(os.path.join(GOLDEN, "reference_vectors.npz"))
     g = _engine(gpu, K.vortex_mesh(8), p)

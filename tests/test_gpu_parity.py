@@ -34,6 +34,50 @@ def test_euler_residual_periodic(gpu, order, tile):
     assert err <= K.STEP_TOL
 
 
+def test_periodic_pairs_recorded_as_interior(gpu):
+    """A conforming periodic face pair is recorded as one interior face (the
+    partner's trace from its coefficients); faces whose images do not land on
+    the partner edge's Gauss points keep the per-point donor lookup."""
+    n = 10
+    g = K.engine_tri(gpu, K.vortex_mesh(n), 2)
+    assert (g.periodic_pairs, g.boundary_faces) == (2 * n, 0)
+    g = K.engine_tri(gpu, K.nonconforming_vortex_mesh(n), 2)
+    assert (g.periodic_pairs, g.boundary_faces) == (n, 2 * n)
+    g = K.engine_tri(gpu, K.rarefaction_mesh(n), 2, boundary=gpu.OUTFLOW)
+    assert (g.periodic_pairs, g.boundary_faces) == (0, 4 * n)
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+def test_euler_residual_periodic_donors(gpu, order, tile):
+    """Periodic faces with per-point donors (the x boundaries of a mesh whose
+    x = 10 nodes are shifted) beside paired ones."""
+    m = K.nonconforming_vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    g = K.engine_tri(gpu, m, order, stage_kernel=tile[0], lanes=tile[1])
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    rng = np.random.default_rng(10 + order)
+    U = U * (1.0 + 1e-3 * rng.standard_normal(U.size))
+    err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
+    print(f"P{order} tile={tile} donor-path residual rel L-inf {err:.3e}")
+    assert err <= K.STEP_TOL
+
+
+def test_advance_periodic_donors_matches_oracle_run(gpu):
+    m = K.nonconforming_vortex_mesh(8)
+    o = K.oracle_tri(m, 2)
+    g = K.engine_tri(gpu, m, 2)
+    U0 = o.project(O.IC_VORTEX, K.VORTEX)
+    end = 12.5 * o.stable_dt(U0, 0.15)
+    Uo, so, to, _ = o.run(U0, 0.15, end, 10 ** 6, 5)
+    g.state = U0
+    tg, sg, _ = g.advance(0.15, end, history_interval=5, history_capacity=64)
+    assert sg == so and abs(tg - to) <= 1e-13 * to
+    err = K.rel_inf_per_variable(g.state, Uo, 4)
+    print(f"donor-path advance {sg} steps rel L-inf {err:.3e}")
+    assert err <= K.RUN_TOL
+
+
 @pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_euler_residual_transmissive(gpu, order, tile):

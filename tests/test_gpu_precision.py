@@ -44,6 +44,20 @@ def test_mixed_residual_periodic(gpu, order):
         assert err <= tol
 
 
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_mixed_residual_periodic_donors(gpu, order):
+    """The matrix-core stage on a mesh whose x boundaries keep per-point
+    periodic donors (the y boundaries are paired as interior faces)."""
+    m = K.nonconforming_vortex_mesh(10)
+    o = K.oracle_tri(m, order)
+    U = o.project(O.IC_VORTEX, K.VORTEX)
+    g = K.engine_tri(gpu, m, order, precision=gpu.MIXED)
+    assert g.boundary_faces == 20
+    err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
+    print(f"P{order} donor-path mixed residual rel L-inf {err:.3e}")
+    assert err <= MIXED_RESIDUAL_TOL
+
+
 @pytest.mark.parametrize("order", [2, 3])
 def test_mixed_residual_transmissive(gpu, order):
     # (the unlimited P1 projection of the rarefaction is unphysical at volume

@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -218,7 +220,7 @@ void choose_tile(int order, int &kernel, int &lanes) {
 struct wasl_hip_engine {
   wasl_hip_numerics num{};
   int dim = 2, p = 0, Np = 1, V = 4;
-  int E = 0, F = 0, NB = 0;
+  int E = 0, F = 0, NB = 0, periodic_pairs = 0;
   int lanes = 1;
   int stage_kernel = 1;       // 1 point split, 2 variable split
   double hmin = 0.0;
@@ -1038,8 +1040,102 @@ void autotune(wasl_hip_engine &g) {
 struct EdgeRecords {
   std::vector<int> nbr, meta, bdon;
   std::vector<double> fn, bbas;
-  int slots = 0;
+  int slots = 0, pairs = 0;
 };
+
+// A periodic boundary face whose donor, at each of its Gauss points, is the
+// element across the opposite boundary edge, at one of that edge's own Gauss
+// points (a conforming periodic mesh): the partner face, the partner's local
+// edge, and the two orderings an interior face records.
+struct PeriodicPair {
+  int partner = -1, kr = 0, flip = 0, rev = 0;
+};
+
+std::vector<PeriodicPair> periodic_pairs(const whip::Mesh &mesh, int p,
+                                         const whip::PeriodicTrace &trace) {
+  const size_t F = mesh.faces.size(), E = mesh.tris.size();
+  const int NE = p + 1;
+  std::vector<PeriodicPair> out(F);
+  if (trace.points != NE || trace.donors.size() != F * NE)
+    return out;
+  double x0 = std::numeric_limits<double>::max(), x1 = std::numeric_limits<double>::lowest();
+  double y0 = x0, y1 = x1;
+  for (const auto &t : mesh.tris)
+    for (int k = 0; k < 3; ++k) {
+      x0 = std::min(x0, t.x[k]);
+      x1 = std::max(x1, t.x[k]);
+      y0 = std::min(y0, t.y[k]);
+      y1 = std::max(y1, t.y[k]);
+    }
+  const double tol = 1e-9 * std::max(x1 - x0, y1 - y0);
+  std::vector<std::array<int, 3>> bface(E, {-1, -1, -1});
+  for (size_t f = 0; f < F; ++f)
+    if (mesh.faces[f].boundary())
+      bface[mesh.faces[f].left][mesh.faces[f].left_edge] = static_cast<int>(f);
+  const auto edges = whip::edge_rules(p);
+  // the image of the face's Gauss point q across the box, as the donor
+  // search places it
+  const auto image = [&](const whip::TriFace &x, int q, double &px, double &py) {
+    mesh.tris[x.left].to_physical(edges[x.left_edge].xi[q], edges[x.left_edge].eta[q], px, py);
+    if (std::fabs(x.nx) > 0.5 && std::fabs(x.ny) < 1e-10)
+      px += x.nx < 0.0 ? x1 - x0 : x0 - x1;
+    else if (std::fabs(x.ny) > 0.5 && std::fabs(x.nx) < 1e-10)
+      py += x.ny < 0.0 ? y1 - y0 : y0 - y1;
+    else
+      px = std::numeric_limits<double>::quiet_NaN();
+  };
+  const auto same_donor = [&](size_t f, int d) {
+    for (int q = 0; q < NE; ++q)
+      if (trace.donors[f * NE + q] != d)
+        return false;
+    return true;
+  };
+  for (size_t f = 0; f < F; ++f) {
+    const whip::TriFace &x = mesh.faces[f];
+    if (!x.boundary())
+      continue;
+    const int d = trace.donors[f * NE];
+    if (d < 0 || d == x.left || !same_donor(f, d))
+      continue;
+    for (int k = 0; k < 3; ++k) {
+      const int g = bface[d][k];
+      if (g < 0 || !same_donor(static_cast<size_t>(g), x.left))
+        continue;
+      const whip::Triangle &dt = mesh.tris[d];
+      int found = -1;
+      for (int flip = 0; flip < 2 && found < 0; ++flip) {
+        bool ok = true;
+        for (int q = 0; q < NE && ok; ++q) {
+          double px, py, rxi, reta, dx, dy;
+          image(x, q, px, py);
+          whip::neighbor_point(x.left_edge, k, flip != 0, edges[x.left_edge].xi[q],
+                               edges[x.left_edge].eta[q], rxi, reta);
+          dt.to_physical(rxi, reta, dx, dy);
+          ok = std::hypot(px - dx, py - dy) <= tol;
+        }
+        if (ok)
+          found = flip;
+      }
+      if (found < 0)
+        continue;
+      int rev = 0;
+      if (NE > 1) {
+        double px, py, r0x, r0y, r1x, r1y;
+        image(x, 0, px, py);
+        dt.to_physical(edges[k].xi[0], edges[k].eta[0], r0x, r0y);
+        dt.to_physical(edges[k].xi[NE - 1], edges[k].eta[NE - 1], r1x, r1y);
+        rev = std::hypot(px - r0x, py - r0y) > std::hypot(px - r1x, py - r1y) ? 1 : 0;
+      }
+      out[f] = {g, k, found, rev};
+      break;
+    }
+  }
+  // only pairs found from both sides
+  for (size_t f = 0; f < F; ++f)
+    if (out[f].partner >= 0 && out[static_cast<size_t>(out[f].partner)].partner != static_cast<int>(f))
+      out[f].partner = -1;
+  return out;
+}
 
 EdgeRecords edge_records(const whip::Mesh &mesh, int p, int Np, const whip::PeriodicTrace *trace) {
   const size_t E = mesh.tris.size();
@@ -1049,6 +1145,8 @@ EdgeRecords edge_records(const whip::Mesh &mesh, int p, int Np, const whip::Peri
   r.meta.assign(3 * E, 0);
   r.fn.assign(9 * E, 0.0);
   const auto edges = whip::edge_rules(p);
+  const std::vector<PeriodicPair> pairs =
+      trace ? periodic_pairs(mesh, p, *trace) : std::vector<PeriodicPair>(mesh.faces.size());
   int slot = 0;
   for (size_t f = 0; f < mesh.faces.size(); ++f) {
     const whip::TriFace &x = mesh.faces[f];
@@ -1067,7 +1165,21 @@ EdgeRecords edge_records(const whip::Mesh &mesh, int p, int Np, const whip::Peri
       r.fn[(3 * k + 1) * E + e] = x.ny;
       r.fn[(3 * k + 2) * E + e] = scale;
     };
-    if (x.boundary()) {
+    const PeriodicPair &pp = pairs[f];
+    if (x.boundary() && pp.partner >= 0) {
+      // A conforming periodic pair becomes an interior face: the face listed
+      // first is its left side, and the stage evaluates the partner's trace
+      // at the left side's points from the partner's coefficients, with the
+      // left normal, as for any interior face.
+      if (pp.partner < static_cast<int>(f))
+        continue;
+      const whip::TriFace &y = mesh.faces[static_cast<size_t>(pp.partner)];
+      const int mp = (x.left_edge << 1) | (pp.kr << 3) | (pp.flip << 5);
+      const int rb = pp.rev ? whip::kMetaReverse : 0;
+      put(x.left, x.left_edge, y.left, mp | whip::kMetaLeft | rb);
+      put(y.left, y.left_edge, x.left, mp | rb);
+      ++r.pairs;
+    } else if (x.boundary()) {
       put(x.left, x.left_edge, -1, m | whip::kMetaLeft | whip::kMetaBoundary | (slot << 8));
       if (trace) {
         for (int q = 0; q < NE; ++q) {
@@ -1213,6 +1325,7 @@ void create_engine(std::shared_ptr<const whip::Mesh> shared, const wasl_hip_nume
     std::vector<int> &nbr = rec.nbr, &meta = rec.meta, &bdon = rec.bdon;
     std::vector<double> &fn = rec.fn, &bbas = rec.bbas;
     g.NB = rec.slots;
+    g.periodic_pairs = rec.pairs;
     upload(g.geo, geo, tot, g.stream);
     upload(g.mass_w, mw, tot, g.stream);
     upload(g.nbr, nbr, tot, g.stream);
@@ -1272,6 +1385,7 @@ void create_engine(std::shared_ptr<const whip::Mesh> shared, const wasl_hip_nume
     g.trace = whip::periodic_trace(mesh, g.p);
     const EdgeRecords rec = edge_records(mesh, g.p, g.Np, &g.trace);
     g.NB = rec.slots;
+    g.periodic_pairs = rec.pairs;
     const double vx = num.wavespeed, vy = num.velocity_y;
     std::vector<double> vel(2 * E), flux(3 * E), mw(E);
     for (size_t e = 0; e < E; ++e) {
@@ -1741,6 +1855,8 @@ int64_t wasl_hip_engine_size(const wasl_hip_engine *g, int what) {
   case 5: return static_cast<int64_t>(g->device_bytes);
   case 6: return g->lanes;
   case 8: return g->stage_kernel;
+  case 9: return g->NB;
+  case 10: return g->periodic_pairs;
   case 7:
     const_cast<wasl_hip_engine *>(g)->build_stencil_host();
     return g->stencil.checks();

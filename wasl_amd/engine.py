@@ -318,6 +318,17 @@ class Engine:
     def device_bytes(self) -> int:
         return int(library().wasl_hip_engine_size(self.h, 5))
 
+    @property
+    def boundary_faces(self) -> int:
+        """Faces kept as boundary faces: transmissive, or periodic with a
+        donor looked up per Gauss point."""
+        return int(library().wasl_hip_engine_size(self.h, 9))
+
+    @property
+    def periodic_pairs(self) -> int:
+        """Conforming periodic face pairs recorded as interior faces."""
+        return int(library().wasl_hip_engine_size(self.h, 10))
+
     def trace(self):
         P = self.order + 1
         d = np.zeros(self.F * P, np.int32)
