@@ -212,14 +212,18 @@ def precision_error(mesh, p: int, precision: int, steps: int, end_time: float = 
     return out
 
 
-def rarefaction_run(cells: int, end_time: float) -> dict:
+def rarefaction_run(cells: int, end_time: float, steps: int = 20, reps: int = 10) -> dict:
     """The radial double rarefaction (rho = 1, p = 0.4, velocity 2 r_hat) on
     [-1,1]^2 with transmissive boundaries, 2 cells^2 jittered triangles, P2,
-    characteristic WENO and positivity limiting (floors 1e-13), CFL 0.1, run
-    to end_time through the FP64 engine. Reports the run's DOF-updates/s
-    (setup excluded) and the smallest cell-average density and pressure at
-    the end, or the reference's stop message if a cell average turns
-    unphysical."""
+    characteristic WENO and positivity limiting (floors 1e-13), CFL 0.1,
+    through the FP64 engine, from the limited projection as the reference
+    composes it (project, limit, advance).
+
+    With these settings the reference itself loses a physical cell average
+    in its 22nd step at 2x256x256 (NaN face fluxes next to the origin), and
+    so does the engine. The run to end_time is therefore reported as its
+    status, and the rate is measured over `reps` runs of the first `steps`
+    limited steps from the same initial state (setup excluded)."""
     import torch
 
     import wasl_amd
@@ -230,26 +234,37 @@ def rarefaction_run(cells: int, end_time: float) -> dict:
                         limiting=True, density_floor=1e-13, pressure_floor=1e-13)
     g.project(wasl_amd.IC_RAREFACTION, (1.0, 0.4, 2.0))
     g.limit()   # the reference limits the projected initial state
+    U0 = g.state
     g.capture_graphs()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     out = {"workload": f"radial double rarefaction, P2, 2x{cells}x{cells}={g.E} jittered "
                        f"triangles on [-1,1]^2, transmissive, char-WENO + positivity (1e-13), "
-                       f"CFL 0.1, T = {end_time}"}
+                       f"CFL 0.1, from the limited projection", "elements": g.E}
     try:
-        t, steps, _ = g.advance(0.1, end_time)
-        out["status"] = "completed"
+        t, n, _ = g.advance(0.1, end_time)
+        out["run_to_end_time"] = {"end_time": end_time, "status": "completed", "steps": n}
     except wasl_amd.SimulationError as exc:
-        t, steps = float("nan"), -1
+        out["run_to_end_time"] = {"end_time": end_time, "status": f"stopped: {exc}"}
+    done = 0
+    wall = 0.0
+    try:
+        for _ in range(reps):
+            g.state = U0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            t, n, _ = g.advance(0.1, 1e9, max_steps=steps, stop_at_max_steps=True)
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
+            done += n
+        out["status"] = f"{reps} runs of {steps} limited steps from the limited projection"
+    except wasl_amd.SimulationError as exc:
         out["status"] = f"stopped: {exc}"
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    out.update({"elements": g.E, "steps": steps, "time": t, "wall_s": wall})
-    if steps > 0:
-        out["dof_updates_per_s"] = g.E * g.Np * 4 * steps / wall
+    out.update({"steps_timed": done, "wall_s": wall})
+    if done > 0 and wall > 0.0:
+        out["dof_updates_per_s"] = g.E * g.Np * 4 * done / wall
         U = g.state.reshape(4, g.Np, g.E)[:, 0] / np.sqrt(2.0)
         rho, mx, my, en = U
         p = 0.4 * (en - 0.5 * (mx * mx + my * my) / rho)
+        out["time_after_steps"] = t
         out["min_cell_density"] = float(rho.min())
         out["min_cell_pressure"] = float(p.min())
     return out
@@ -570,7 +585,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", choices=["vortex", "rarefaction", "advect2d", "sod"],
                     default="vortex",
                     help="vortex: the headline (config 2 sizes by default); rarefaction: the "
-                         "limited radial rarefaction run to --end-time; advect2d: 2D scalar "
+                         "limited radial rarefaction (its run to --end-time reported, 10 x 20 "
+                         "limited steps timed); advect2d: 2D scalar "
                          "advection at --order and --cells; sod: 1D Euler Sod tube, --cells "
                          "cells; each other workload prints one JSON line of its own")
     ap.add_argument("--end-time", type=float, default=0.3)
