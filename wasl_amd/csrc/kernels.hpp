@@ -876,12 +876,24 @@ void euler_stage(EulerMesh mesh, StageArgs a) {
 constexpr int kVTableMax = 3 * 40 * 10;   // Tri<3>::kVTable
 __constant__ double g_vtable[4][kVTableMax];
 
-// Orders LDS traffic between the lanes of one wave (all lanes of an element
-// live in the same wave, so no block barrier is needed).
-__device__ __forceinline__ void lane_exchange_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+// 4x4 transpose across the four lanes of each quad: lane v enters with row v
+// (x[j] = M[v][j]) and leaves with column v (x[j] = M[j][v]). Partners two
+// lanes apart swap the off-diagonal 2x2 blocks, then partners one lane apart
+// swap within them. Every lane of the quad must be active.
+__device__ __forceinline__ void quad_transpose(double (&x)[4]) {
+  const int v = static_cast<int>(threadIdx.x) & 3;
+  const bool b1 = (v & 2) != 0, b0 = (v & 1) != 0;
+  double a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double t = dpp<kQuadXor2>(x[j ^ 2]);
+    a[j] = (((j & 2) != 0) == b1) ? x[j] : t;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double t = dpp<kQuadXor1>(a[j ^ 1]);
+    x[j] = (((j & 1) != 0) == b0) ? a[j] : t;
+  }
 }
 
 // The same fused stage with a different work split: each element is served
@@ -889,16 +901,15 @@ __device__ __forceinline__ void lane_exchange_fence() {
 // coefficients and Np right-hand-side accumulators -- and the quads of an
 // element take interleaved groups of four volume points and whole edges.
 // States at points and point fluxes move between the four lanes of a quad
-// through a small LDS scratch, so no lane holds another variable's modes and
-// no cross-lane reduction of the accumulators is needed (a single one across
-// the H quads when H = 2). Volume tables are wave-uniform and read through
-// the scalar cache; edge tables depend on each element's face configuration
-// and are staged in LDS.
-// LDS of one variable-split workgroup. The exchange arrays come first and
-// 16-byte aligned, so a lane's four variables move as one 128-bit access.
+// by 4x4 transposes in DPP quad permutes (no LDS round trip), so no lane
+// holds another variable's modes and no cross-lane reduction of the
+// accumulators is needed (a single one across the H quads when H = 2).
+// Scheduling barriers at the exchange points keep the compiler from
+// stretching live ranges across them (without them P3 spilled). Volume
+// tables are wave-uniform and read through the scalar cache; edge tables
+// depend on each element's face configuration and are staged in LDS.
+// LDS of one variable-split workgroup.
 template <int P> struct alignas(16) VsShared {
-  alignas(16) double xs[64][2][4][4];   // per quad: [traces or states][point][variable]
-  alignas(16) double xf[64][4][8];      // per quad: [point][F_xi 0..3, F_eta 0..3] or [point][F 0..3]
   alignas(16) double tab[Tri<P>::kEdgeTable];
   double red[4];
 };
@@ -914,8 +925,6 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
   constexpr int PER_BLOCK = 64 / H;
   constexpr int NIT = (D::NQ + 4 * H - 1) / (4 * H);
   double *tab = sh.tab;
-  auto &xs = sh.xs;
-  auto &xf = sh.xf;
   double *red = sh.red;
 
   const int E = mesh.E;
@@ -995,28 +1004,30 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
 
   static_assert(D::kVTable <= kVTableMax, "volume table slot too small");
   const double *vt = g_vtable[P];
-  double(*xq)[4] = xs[quad][0];
-  double(*xo)[4] = xs[quad][1];
-  double(*ff)[8] = xf[quad];
 
   // Volume integral, four points per quad and iteration.
   if constexpr (P > 0) {
 #pragma unroll 1
     for (int it = 0; it < NIT; ++it) {
       const int q0 = (it * H + h) * 4;
+      // x[j]: variable v at point q0 + j; after the transpose, x[k]: variable k
+      // at point q0 + v
+      double x[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double *b = vt + D::kVB + (q0 + j) * Np;
-        double x = 0.0;
+        double t = 0.0;
 #pragma unroll
         for (int m = 0; m < Np; ++m)
-          x += c[m] * b[m];
-        xq[j][v] = x;
+          t += c[m] * b[m];
+        x[j] = t;
       }
-      lane_exchange_fence();
+      quad_transpose(x);
+      __builtin_amdgcn_sched_barrier(0);
+      double fxi4[4], feta4[4];
       {
         // lane v evaluates the flux at point q0 + v
-        const double s0 = xq[v][0], s1 = xq[v][1], s2 = xq[v][2], s3 = xq[v][3];
+        const double s0 = x[0], s1 = x[1], s2 = x[2], s3 = x[3];
         const double inv = rcp(s0);
         const double u = s1 * inv, w = s2 * inv;
         const double p = g1 * (s3 - 0.5 * (s1 * u + s2 * w));
@@ -1024,14 +1035,17 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
         const double fy[4] = {s2, s1 * w, s2 * w + p, (s3 + p) * w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          ff[v][k] = gA * fx[k] + gB * fy[k];
-          ff[v][4 + k] = gC * fx[k] + gD * fy[k];
+          fxi4[k] = gA * fx[k] + gB * fy[k];
+          feta4[k] = gC * fx[k] + gD * fy[k];
         }
       }
-      lane_exchange_fence();
+      // fxi4[j], feta4[j]: variable v's reference fluxes at point q0 + j
+      quad_transpose(fxi4);
+      quad_transpose(feta4);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double fxi = ff[j][v], feta = ff[j][4 + v];
+        const double fxi = fxi4[j], feta = feta4[j];
         const double *wx = vt + D::kVWx + (q0 + j) * Np;
         const double *wy = vt + D::kVWy + (q0 + j) * Np;
 #pragma unroll
@@ -1042,7 +1056,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
             acc[m] = fma(wy[m], feta, acc[m]);
         }
       }
-      lane_exchange_fence();
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 
@@ -1060,6 +1074,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
     const double *br = tab + (D::kBr - D::kEdge) + ((kl * 3 + kr) * 2 + fl) * NE * Np;
     const double *own = is_left ? bl : br;
     const double *oth = is_left ? br : bl;
+    double si4[4] = {0.0, 0.0, 0.0, 0.0}, so4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
       double si = 0.0;
@@ -1087,37 +1102,42 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
           so = si;  // transmissive: exterior = own trace
         }
       }
-      xq[q][v] = si;
-      xo[q][v] = so;
+      si4[q] = si;
+      so4[q] = so;
     }
     if (next >= 0) {
 #pragma unroll
       for (int m = 0; m < Np; ++m)
         nv[m] = at_idx(a.uin, static_cast<unsigned>((v * Np + m) * E + next));
     }
-    lane_exchange_fence();
+    // after the transposes lane v holds all four variables at edge point v
+    quad_transpose(si4);
+    quad_transpose(so4);
+    __builtin_amdgcn_sched_barrier(0);
+    double fv[4] = {0.0, 0.0, 0.0, 0.0};
     if (v < NE) {
-      const C4 si = {xq[v][0], xq[v][1], xq[v][2], xq[v][3]};
-      const C4 so = {xo[v][0], xo[v][1], xo[v][2], xo[v][3]};
+      const C4 si = {si4[0], si4[1], si4[2], si4[3]};
+      const C4 so = {so4[0], so4[1], so4[2], so4[3]};
       const C4 L = is_left ? si : so;
       const C4 R = is_left ? so : si;
       const C4 f = hllc(g, L, R, nx, ny);
       const double wq = tab[(D::kWe - D::kEdge) + kl * NE + v] * scale;
       const double sw = is_left ? -wq : wq;
-      ff[v][0] = f.r * sw;
-      ff[v][1] = f.mx * sw;
-      ff[v][2] = f.my * sw;
-      ff[v][3] = f.E * sw;
+      fv[0] = f.r * sw;
+      fv[1] = f.mx * sw;
+      fv[2] = f.my * sw;
+      fv[3] = f.E * sw;
     }
-    lane_exchange_fence();
+    // fv[q]: variable v's weighted flux at edge point q
+    quad_transpose(fv);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
-      const double val = ff[q][v];
+      const double val = fv[q];
 #pragma unroll
       for (int m = 0; m < Np; ++m)
         acc[m] += val * own[q * Np + m];
     }
-    lane_exchange_fence();
   };
   if constexpr (KP == KE) {
 #pragma unroll
