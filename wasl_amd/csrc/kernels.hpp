@@ -2145,6 +2145,12 @@ __device__ __forceinline__ void line_body(const LineMeshDev mesh, const StageArg
   const double fr = upwind(sa, own_r, ext_r);   // right face of e
   if (valid) {
     const double ij = mesh.inv_jac[e];
+    // U^n first, all of it: a read issued after a store would wait for that
+    // store as well.
+    double un[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      un[m] = (!a.residual_only && a.ca != 0.0) ? a.un[e * Np + m] : 0.0;
 #pragma unroll
     for (int m = 0; m < Np; ++m) {
       const double r = (acc[m] + bl[m] * fl - br[m] * fr) * ij;
@@ -2154,7 +2160,7 @@ __device__ __forceinline__ void line_body(const LineMeshDev mesh, const StageArg
       } else {
         y = a.cb * (c[m] + dt * r);
         if (a.ca != 0.0)
-          y += a.ca * a.un[e * Np + m];
+          y += a.ca * un[m];
       }
       a.out[e * Np + m] = y;
     }
@@ -2350,6 +2356,17 @@ __device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageA
   edge(meta[1], fe[1], nbk);
   edge(meta[2], fe[2], na);
 
+  // U^n first, all of it: a read issued after a store would wait for that
+  // store as well.
+  double un[Np];
+#pragma unroll
+  for (int m = 0; m < Np; ++m)
+    un[m] = 0.0;
+  if (!a.residual_only && a.ca != 0.0) {
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      un[m] = a.un[static_cast<size_t>(m) * static_cast<unsigned>(E) + e];
+  }
 #pragma unroll
   for (int m = 0; m < Np; ++m) {
     const size_t idx = static_cast<size_t>(m) * static_cast<unsigned>(E) + e;
@@ -2359,7 +2376,7 @@ __device__ __forceinline__ void advect_body(const AdvectMesh &mesh, const StageA
     } else {
       y = a.cb * (c[m] + dt * acc[m]);
       if (a.ca != 0.0)
-        y += a.ca * a.un[idx];
+        y += a.ca * un[m];
     }
     if (valid)
       a.out[idx] = y;
@@ -2483,6 +2500,14 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
   const C4 fr = hllc(g, own_r, ext_r, 1.0, 0.0);
   const double vl[4] = {fl.r, fl.mx, fl.my, fl.E}, vr[4] = {fr.r, fr.mx, fr.my, fr.E};
   const double ij = mesh.inv_jac[e];
+  // U^n first, all of it: a read issued after a store would wait for that
+  // store as well.
+  double un[4][Np];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      un[v][m] = (!a.residual_only && a.ca != 0.0) ? a.un[(v * Np + m) * E + e] : 0.0;
   double out0[4], dens[Np];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -2496,7 +2521,7 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
       } else {
         y = a.cb * (c[v][m] + dt * r);
         if (a.ca != 0.0)
-          y += a.ca * a.un[idx];
+          y += a.ca * un[v][m];
       }
       if (valid)
         a.out[idx] = y;
