@@ -2951,36 +2951,31 @@ __device__ __forceinline__ void limit_wave_body(const LimiterDev L, const double
 
 // Writes the limited coefficients back; with a trace buffer (mixed
 // precision) also the FP32 edge traces of the limited element.
-template <int Np>
-__device__ __forceinline__ bool scatter_core(double *U, int E, const double *side, const int *list,
-                                             const int *count, RunCtl *ctl, int parity, int idx,
-                                             const int *fail = nullptr) {
-  if (ctl && (ctl->err || !ctl->s[parity].active))
-    return false;
-  if (idx >= *count)
-    return false;
-  const int e = list[idx];
-  if (fail && e >= *fail)
-    return false;   // at or after the first element whose average is below a floor
-  const double *src = side + static_cast<size_t>(idx) * 4 * Np;
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int m = 0; m < Np; ++m)
-      U[(v * Np + m) * E + e] = src[v * Np + m];
-  return true;
-}
-
 template <int P>
 __device__ __forceinline__ void scatter_body(double *U, int E, const double *side, const int *list,
                                              const int *count, RunCtl *ctl, int parity, int idx,
                                              float *tr = nullptr, const float *mxtab = nullptr,
                                              const int *fail = nullptr) {
   constexpr int Np = Tri<P>::Np;
-  if (!scatter_core<Np>(U, E, side, list, count, ctl, parity, idx, fail))
+  if (ctl && (ctl->err || !ctl->s[parity].active))
+    return;
+  if (idx >= *count)
     return;
   const int e = list[idx];
+  if (fail && e >= *fail)
+    return;   // at or after the first element whose average is below a floor
+  // Every value is read before the first store: the compiler cannot tell U
+  // from side apart, and a read after a store to U would wait for it.
   const double *src = side + static_cast<size_t>(idx) * 4 * Np;
+  double val[4 * Np];
+#pragma unroll
+  for (int i = 0; i < 4 * Np; ++i)
+    val[i] = src[i];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int m = 0; m < Np; ++m)
+      U[(v * Np + m) * E + e] = val[v * Np + m];
   if (tr) {
     using X = Mx<P>;
     constexpr int NE = X::NE;
@@ -2993,7 +2988,7 @@ __device__ __forceinline__ void scatter_body(double *U, int E, const double *sid
           for (int m = 0; m < Np; ++m)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-              x[v] += static_cast<float>(src[v * Np + m]) * b[m];
+              x[v] += static_cast<float>(val[v * Np + m]) * b[m];
         }
         reinterpret_cast<float4 *>(tr + (static_cast<size_t>(e) * 3 + k) * 16)[q] =
             make_float4(x[0], x[1], x[2], x[3]);
@@ -3024,11 +3019,31 @@ __device__ __forceinline__ void scatter_strided(double *U, int E, const double *
   const int n = *count * 4 * Np;
   const int first_bad = fail ? *fail : INT_MAX;
   const int stride = static_cast<int>(gridDim.x * blockDim.x);
-  for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
-    const int slot = i / (4 * Np), r = i - slot * 4 * Np;
-    const int e = list[slot];
-    if (e < first_bad)
-      U[static_cast<size_t>(r) * static_cast<unsigned>(E) + e] = side[i];
+  // Four values per thread and pass, all read before any is stored: the
+  // compiler cannot tell U from side apart, and a read after a store to U
+  // would wait for it.
+  constexpr int kPer = 4;
+  for (int i0 = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i0 < n;
+       i0 += kPer * stride) {
+    int row[kPer], el[kPer];
+    double x[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = i0 + k * stride;
+      el[k] = -1;
+      row[k] = 0;
+      x[k] = 0.0;
+      if (i < n) {
+        const int slot = i / (4 * Np);
+        row[k] = i - slot * 4 * Np;
+        el[k] = list[slot];
+        x[k] = side[i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+      if (el[k] >= 0 && el[k] < first_bad)
+        U[static_cast<size_t>(row[k]) * static_cast<unsigned>(E) + el[k]] = x[k];
   }
 }
 
