@@ -1766,7 +1766,10 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
 
   // ---- finalise: lane holds R[v][row 4 grp + r] of element j, mode KS grp + r
   // for r < KS; FP64 from here
+  // Every output value is formed before the first store: a store issued
+  // between the waits for U^n would make the next wait include it.
   float yf[4][KS];
+  double xo[4][KS];
   double avg[4] = {0.0, 0.0, 0.0, 0.0};
   double total = 0.0, high = 0.0;
   const Rsrc rout = rsrc(a.out, state_bytes);
@@ -1776,6 +1779,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     for (int r = 0; r < KS; ++r) {
       const int m = KS * grp + r;
       yf[v][r] = 0.0f;
+      xo[v][r] = 0.0;
       if (m < Np) {
         const double res = (static_cast<double>(R[v][r]) + vmean[v][r]) * invdet;
         double x;
@@ -1786,8 +1790,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
           if (a.ca != 0.0)
             x += a.ca * un64[v][r];
         }
-        if (valid)
-          st_f64(x, rout, goff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
+        xo[v][r] = x;
         yf[v][r] = static_cast<float>(x);
         if (r == 0)
           avg[v] = x;
@@ -1798,6 +1801,14 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
         }
       }
     }
+  }
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < KS; ++r)
+        if (KS * grp + r < Np)
+          st_f64(xo[v][r], rout, goff, static_cast<unsigned>(v * Np + r) * Eu * 8u);
   }
 
   if (a.tout) {
