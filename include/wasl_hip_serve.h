@@ -80,10 +80,13 @@ int wasl_hip_server_submit(wasl_hip_server *server, const wasl_hip_mesh *mesh,
                            int64_t param_count, double cfl, double end_time, uint64_t end_steps,
                            uint64_t max_steps, uint64_t *id);
 
-/* One scheduling round: start preparing queued cases that fit the budget,
- * admit prepared ones, advance the resident set by up to steps_per_batch steps
- * and retire finished cases. Returns up to capacity finished request ids. When
- * nothing is resident it waits (up to 50 ms) for a preparation to finish. */
+/* One scheduling round, pipelined with the device: wait for the round the
+ * previous call launched and take its finished cases out of the resident set,
+ * start preparing queued cases that fit the budget, admit prepared ones,
+ * launch the next round of up to steps_per_batch steps without waiting for it,
+ * and retire the finished cases (state copied out, engine released) while it
+ * runs. Returns up to capacity finished request ids. When nothing is resident
+ * it waits (up to 50 ms) for a preparation to finish. */
 int wasl_hip_server_pump(wasl_hip_server *server, uint64_t *completed, int64_t capacity,
                          int64_t *count);
 

@@ -15,7 +15,7 @@
 
 namespace {
 
-enum class Phase { queued, preparing, ready, resident, done };
+enum class Phase { queued, preparing, ready, resident, finishing, done };
 
 struct Request {
   uint64_t id = 0;
@@ -34,6 +34,15 @@ struct Request {
   std::vector<double> result;
   double time = 0.0;
   uint64_t step = 0;
+};
+
+// A request that finished in a round: its outcome, and where its state sits
+// in the server's pinned staging.
+struct Finish {
+  Request *r;
+  int status;
+  std::string message;
+  size_t offset;
 };
 
 // Device bytes of an engine for this case before it exists (state, stage
@@ -113,6 +122,7 @@ struct wasl_hip_server {
   std::vector<uint64_t> resident;    // in admission order
   std::vector<uint64_t> finished;    // done and unclaimed
   bool dirty = true;
+  bool in_flight = false;            // a round launched by the previous pump runs
   int64_t graphs = 0, rounds = 0, changes = 0;
   int64_t rebuild_us = 0, wait_us = 0, admit_us = 0, retire_us = 0, idle_us = 0, submit_us = 0;
   int64_t release_us = 0;
@@ -129,6 +139,8 @@ struct wasl_hip_server {
   whip::StepSlot *slots_host = nullptr;   // pinned
   int *errs_host = nullptr;               // pinned
   size_t slots_capacity = 0;
+  double *stage_host = nullptr;           // pinned staging of finished states
+  size_t stage_capacity = 0;              // doubles
 
   // Every C ABI call on the server holds api_mu for its duration, so calls
   // from several threads are serialised (a submit from another thread waits
@@ -159,6 +171,8 @@ struct wasl_hip_server {
       (void)hipHostFree(slots_host);
     if (errs_host)
       (void)hipHostFree(errs_host);
+    if (stage_host)
+      (void)hipHostFree(stage_host);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -555,11 +569,12 @@ __global__ void gather_slots(whip::RunCtl *const *ctls, int n, whip::StepSlot *o
   }
 }
 
-void retire(wasl_hip_server &s, Request &r, int status, const std::string &message) {
+// Finishes a request whose final state is already on the host (`state`,
+// the engine's state_size() doubles): keeps it and releases the engine.
+void retire(wasl_hip_server &s, Request &r, int status, const std::string &message,
+            const double *state) {
   wasl_hip_engine &g = *r.eng;
-  r.result.resize(g.state_size());
-  HIPCHECK(hipMemcpy(r.result.data(), g.u.p, r.result.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
+  r.result.assign(state, state + g.state_size());
   r.status = status;
   r.message = message;
   const auto t_release = std::chrono::steady_clock::now();
@@ -693,11 +708,52 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
     if (!s)
       throw std::runtime_error("server: null argument");
     std::lock_guard<std::mutex> api(s->api_mu);
+    using clock = std::chrono::steady_clock;
+    const auto us = [](clock::time_point a, clock::time_point b) {
+      return std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
+    };
     std::vector<uint64_t> done_now, ready_now;
+    // Requests that finished in the round just completed. They leave the
+    // resident set before the next round is launched and are retired (state
+    // copied out, engine released) while that round runs.
+    std::vector<Finish> finishing;
+    if (s->in_flight) {
+      const auto t_wait = clock::now();
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      s->wait_us += us(t_wait, clock::now());
+      s->in_flight = false;
+      const int n = static_cast<int>(s->ctl_ids.size());
+      for (int i = 0; i < n; ++i) {
+        Request &r = *s->requests.at(s->ctl_ids[i]);
+        const whip::StepSlot &st = s->slots_host[i];
+        r.time = st.time;
+        r.step = st.step;
+        if (s->errs_host[i]) {
+          finishing.push_back({&r, WASL_HIP_SIMULATION, device_error_text(s->errs_host[i]), 0});
+        } else if (!(st.time < r.end_time) || st.step >= r.max_steps) {
+          const bool short_of_end = st.time < r.end_time;
+          finishing.push_back({&r, short_of_end ? WASL_HIP_SIMULATION : WASL_HIP_OK,
+                               short_of_end ? "simulation: max_steps reached before end_time"
+                                            : "",
+                               0});
+        }
+      }
+      if (!finishing.empty()) {
+        for (Finish &f : finishing)
+          f.r->phase = Phase::finishing;
+        // keep admission order among the remaining cases
+        std::vector<uint64_t> keep;
+        for (uint64_t id : s->resident)
+          if (s->requests.at(id)->phase == Phase::resident)
+            keep.push_back(id);
+        s->resident.swap(keep);
+        s->dirty = true;
+      }
+    }
     {
       std::unique_lock<std::mutex> lock(s->mu);
       // start preparations in arrival order while the budget and slots allow
-      size_t active = s->resident.size();
+      size_t active = s->resident.size() + finishing.size();
       for (auto &[id, r] : s->requests)
         if (r->phase == Phase::preparing || r->phase == Phase::ready)
           ++active;
@@ -714,7 +770,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
         ++active;
       }
       s->cv_work.notify_all();
-      if (s->resident.empty()) {
+      if (s->resident.empty() && finishing.empty()) {
         const auto has_ready = [&] {
           for (auto &[id, r] : s->requests)
             if (r->phase == Phase::ready || (r->phase == Phase::done && r->eng == nullptr &&
@@ -722,11 +778,9 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
               return true;
           return false;
         };
-        const auto t_idle = std::chrono::steady_clock::now();
+        const auto t_idle = clock::now();
         s->cv_ready.wait_for(lock, std::chrono::milliseconds(50), has_ready);
-        s->idle_us += std::chrono::duration_cast<std::chrono::microseconds>(
-                          std::chrono::steady_clock::now() - t_idle)
-                          .count();
+        s->idle_us += us(t_idle, clock::now());
       }
       // failed preparations finish here; ready ones are admitted
       for (auto &[id, r] : s->requests) {
@@ -740,7 +794,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
         }
       }
     }
-    const auto t_admit = std::chrono::steady_clock::now();
+    const auto t_admit = clock::now();
     for (uint64_t id : ready_now) {
       Request *r = s->requests.at(id).get();
       // replace the estimate by what the engine holds
@@ -750,14 +804,8 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
       s->resident.push_back(id);
       s->dirty = true;
     }
-    s->admit_us += std::chrono::duration_cast<std::chrono::microseconds>(
-                       std::chrono::steady_clock::now() - t_admit)
-                       .count();
+    s->admit_us += us(t_admit, clock::now());
     if (!s->resident.empty()) {
-      using clock = std::chrono::steady_clock;
-      const auto us = [](clock::time_point a, clock::time_point b) {
-        return std::chrono::duration_cast<std::chrono::microseconds>(b - a).count();
-      };
       const auto t_build = clock::now();
       if (s->dirty) {
         rebuild(*s);
@@ -768,7 +816,7 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
       s->rebuild_us += us(t_build, clock::now());
       ++s->rounds;
       // gather the step slots (parity 0 after an even round) and error codes,
-      // in the order of the launch tables
+      // in the order of the launch tables; read at the next pump
       const int n = static_cast<int>(s->ctl_list.size());
       hipLaunchKernelGGL(gather_slots, dim3((n + 63) / 64), dim3(64), 0, s->stream,
                          s->ctl_ptrs.as<whip::RunCtl *>(), n, s->ctl_out.as<whip::StepSlot>(),
@@ -778,36 +826,37 @@ int wasl_hip_server_pump(wasl_hip_server *s, uint64_t *completed, int64_t capaci
                               hipMemcpyDeviceToHost, s->stream));
       HIPCHECK(hipMemcpyAsync(s->errs_host, s->err_out.p, n * sizeof(int), hipMemcpyDeviceToHost,
                               s->stream));
-      const auto t_wait = clock::now();
-      HIPCHECK(hipStreamSynchronize(s->stream));
-      s->wait_us += us(t_wait, clock::now());
+      s->in_flight = true;
+    }
+    // The finished requests are in no launch table of the running round.
+    // Their states come out through pinned staging on each engine's own
+    // stream (non-blocking and otherwise idle), beside the round. (Leaving
+    // the copies in flight until the next call measured no faster.)
+    if (!finishing.empty()) {
       const auto t_retire = clock::now();
-      std::vector<uint64_t> still;
-      for (int i = 0; i < n; ++i) {
-        Request &r = *s->requests.at(s->ctl_ids[i]);
-        const whip::StepSlot &st = s->slots_host[i];
-        r.time = st.time;
-        r.step = st.step;
-        if (s->errs_host[i]) {
-          retire(*s, r, WASL_HIP_SIMULATION, device_error_text(s->errs_host[i]));
-          done_now.push_back(r.id);
-        } else if (!(st.time < r.end_time) || st.step >= r.max_steps) {
-          const bool short_of_end = st.time < r.end_time;
-          retire(*s, r, short_of_end ? WASL_HIP_SIMULATION : WASL_HIP_OK,
-                 short_of_end ? "simulation: max_steps reached before end_time" : "");
-          done_now.push_back(r.id);
-        } else {
-          still.push_back(r.id);
-        }
+      size_t total = 0;
+      for (const Finish &f : finishing)
+        total += f.r->eng->state_size();
+      if (s->stage_capacity < total) {
+        if (s->stage_host)
+          HIPCHECK(hipHostFree(s->stage_host));
+        s->stage_host = nullptr;
+        s->stage_capacity = std::max(total, 2 * s->stage_capacity);
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&s->stage_host),
+                               s->stage_capacity * sizeof(double)));
       }
-      if (still.size() != s->resident.size()) {
-        // keep admission order among the remaining cases
-        std::vector<uint64_t> keep;
-        for (uint64_t id : s->resident)
-          if (s->requests.at(id)->phase == Phase::resident)
-            keep.push_back(id);
-        s->resident.swap(keep);
-        s->dirty = true;
+      size_t off = 0;
+      for (Finish &f : finishing) {
+        const wasl_hip_engine &g = *f.r->eng;
+        HIPCHECK(hipMemcpyAsync(s->stage_host + off, g.u.p, g.state_size() * sizeof(double),
+                                hipMemcpyDeviceToHost, g.stream));
+        f.offset = off;
+        off += g.state_size();
+      }
+      for (Finish &f : finishing) {
+        HIPCHECK(hipStreamSynchronize(f.r->eng->stream));
+        retire(*s, *f.r, f.status, f.message, s->stage_host + f.offset);
+        done_now.push_back(f.r->id);
       }
       s->retire_us += us(t_retire, clock::now());
     }
