@@ -1445,12 +1445,15 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
       c64[v][r] = real ? x : 0.0;
       cf[v][r] = static_cast<float>(c64[v][r]);
     }
-  constexpr int kPer = (kStaged + 255) / 256;
-  float tpre[kPer];
+  // the operand table in 16-byte pieces (its length is a multiple of 4)
+  static_assert(kStaged % 4 == 0, "operand table in 16-byte pieces");
+  constexpr int kPer = (kStaged / 4 + 255) / 256;
+  float4 tpre[kPer];
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = static_cast<int>(threadIdx.x) + 256 * i;
-    tpre[i] = idx < kStaged ? mxtab[idx] : 0.0f;
+    tpre[i] = idx < kStaged / 4 ? reinterpret_cast<const float4 *>(mxtab)[idx]
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   const Rsrc rgeo = rsrc(mesh.geo, static_cast<size_t>(5) * Eu * 8);
   const double invdet = ld_f64(rgeo, eoff, 4u * Eu * 8u);
@@ -1491,8 +1494,8 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int idx = static_cast<int>(threadIdx.x) + 256 * i;
-    if (idx < kStaged)
-      tab[idx] = tpre[i];
+    if (idx < kStaged / 4)
+      reinterpret_cast<float4 *>(tab)[idx] = tpre[i];
   }
   __syncthreads();
 
