@@ -1473,7 +1473,9 @@ void create_engine(std::shared_ptr<const whip::Mesh> shared, const wasl_hip_nume
   HIPCHECK(hipMemsetAsync(g.counters.p, 0, g.counters.bytes, g.stream));
   g.err.alloc(2 * sizeof(int), tot, g.stream);
   reset_device_error(g);
-  g.ctl.alloc(sizeof(whip::RunCtl), tot, g.stream);
+  // the step control and, after it, the slots its stages spread the CFL speed over
+  g.ctl.alloc(whip::kCtlBytes, tot, g.stream);
+  HIPCHECK(hipMemsetAsync(g.ctl.p, 0, g.ctl.bytes, g.stream));
   g.partials.alloc(2 * sizeof(double) * ((g.E + 255) / 256), tot, g.stream);
   g.result.alloc(sizeof(double), tot, g.stream);
   HIPCHECK(hipStreamSynchronize(g.stream));

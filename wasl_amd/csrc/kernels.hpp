@@ -99,6 +99,8 @@ struct StepSlot {
   unsigned long long lambda;  // max |u|+c over cell averages (bits of a positive double)
   int active;
   int bad;                    // an unphysical cell average was seen
+  int spread;                 // 1: the speed is the max of lambda and this parity's spread slots
+  int pad_;
 };
 
 struct RunCtl {
@@ -111,6 +113,23 @@ struct RunCtl {
   double last_dt;             // dt of the last step taken
   StepSlot s[2];
 };
+
+// The CFL speed of a step's new state is a max over every workgroup of its
+// last stage. All of them updating one address serialises the updates at
+// the end of the stage, so they are spread over kSpeedSlots slots per step
+// parity, one 128-byte line each (workgroup b to slot b % kSpeedSlots), kept
+// in the step control's device allocation after the RunCtl itself. The next
+// step's first stage takes the max of the slots.
+constexpr int kSpeedSlots = 16;
+constexpr int kSpeedStride = 16;   // unsigned long longs: 128 bytes
+constexpr size_t kSpeedOffset = (sizeof(RunCtl) + 255) / 256 * 256;
+constexpr size_t kCtlBytes =
+    kSpeedOffset + 2 * kSpeedSlots * kSpeedStride * sizeof(unsigned long long);
+
+__device__ __forceinline__ unsigned long long *speed_slots(RunCtl *ctl, int parity) {
+  return reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(ctl) + kSpeedOffset) +
+         parity * kSpeedSlots * kSpeedStride;
+}
 
 struct EulerMesh {
   int E;
@@ -179,6 +198,39 @@ __device__ __forceinline__ void raise_error(RunCtl *ctl, int *err, int code) {
   int *slot = ctl ? &ctl->err : err;
   if (slot)
     atomicCAS(slot, 0, code);
+}
+
+// Speeds are non-negative doubles, which order like their bit patterns.
+__device__ __forceinline__ void speed_update(RunCtl *ctl, int parity, double speed) {
+  atomicMax(speed_slots(ctl, parity) + (blockIdx.x % kSpeedSlots) * kSpeedStride,
+            static_cast<unsigned long long>(__double_as_longlong(speed)));
+}
+
+template <int CTRL> __device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(x >> 32), CTRL, 0xF, 0xF, false);
+  return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+         static_cast<unsigned>(lo);
+}
+
+// Lane l's share of the spread slots of a step parity: slot l % 16.
+__device__ __forceinline__ unsigned long long speed_slot_of_lane(RunCtl *ctl, int parity) {
+  static_assert(kSpeedSlots == 16, "one slot per lane of a row");
+  return speed_slots(ctl, parity)[(static_cast<int>(threadIdx.x) & 15) * kSpeedStride];
+}
+
+// The max of the slots the lanes read: each 16-lane row reduces by quad
+// permutes and row rotations, and the result is wave-uniform. Every lane
+// must be active.
+__device__ __forceinline__ unsigned long long speed_slots_max(unsigned long long x) {
+  const auto mx = [](unsigned long long a, unsigned long long b) { return a < b ? b : a; };
+  x = mx(x, dpp_u64<kQuadXor1>(x));
+  x = mx(x, dpp_u64<kQuadXor2>(x));
+  x = mx(x, dpp_u64<0x124>(x));   // row_ror:4
+  x = mx(x, dpp_u64<0x128>(x));   // row_ror:8
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<int>(x));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
 // Ideal-gas pointwise physics (equations/euler/primitive.hpp, flux.hpp).
@@ -269,6 +321,8 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
   const double order_factor = ctl->order_factor;
   const unsigned long long max_steps = ctl->max_steps;
   const int fixed_speed = ctl->fixed_speed;
+  const int spread = slot.spread;
+  const unsigned long long part = stage == 0 ? speed_slot_of_lane(ctl, parity) : 0ull;
   if (err)
     return false;
   if (stage != 0) {
@@ -277,6 +331,10 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
     dt = slot_dt;
     return true;
   }
+  // the speed of the step's start: lambda, with the slots the previous
+  // step's last stage filled when it gathered the speed there
+  const unsigned long long slots = speed_slots_max(part);
+  const unsigned long long lam = spread && slots > lambda ? slots : lambda;
   const bool active = time < end_time && step < max_steps;
   if (!active) {
     if (leader && threadIdx.x == 0) {
@@ -284,7 +342,8 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
       StepSlot &nxt = ctl->s[parity ^ 1];
       nxt.time = time;
       nxt.step = step;
-      nxt.lambda = lambda;
+      nxt.lambda = lam;
+      nxt.spread = 0;
       nxt.bad = bad;
       nxt.active = 0;
     }
@@ -295,7 +354,7 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
       raise_error(ctl, nullptr, kErrUnphysical);
     return false;
   }
-  const double speed = __longlong_as_double(static_cast<long long>(lambda));
+  const double speed = __longlong_as_double(static_cast<long long>(lam));
   const double remaining = end_time - time;
   const double stable = speed > 0.0 ? cfl_length / (speed * order_factor) : 0.0;
   dt = fmin(stable > 0.0 ? stable : remaining, remaining);
@@ -309,9 +368,13 @@ __device__ inline bool stage_prologue(RunCtl *ctl, int parity, int stage, double
     slot.active = 1;
     ctl->last_dt = dt;
     StepSlot &nxt = ctl->s[parity ^ 1];
-    nxt.lambda = fixed_speed ? lambda : 0ull;
+    nxt.lambda = fixed_speed ? lam : 0ull;
+    nxt.spread = fixed_speed ? 0 : 1;
     nxt.bad = 0;
   }
+  // the next step's slots start empty (its last stage fills them)
+  if (leader && !fixed_speed && threadIdx.x < kSpeedSlots)
+    speed_slots(ctl, parity ^ 1)[threadIdx.x * kSpeedStride] = 0ull;
   return true;
 }
 
@@ -855,8 +918,7 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
     if (threadIdx.x == 0) {
       const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
       if (ctl)
-        atomicMax(&ctl->s[a.parity ^ 1].lambda,
-                  static_cast<unsigned long long>(__double_as_longlong(mx)));
+        speed_update(ctl, a.parity ^ 1, mx);
     }
   }
   stage_epilogue(ctl, a.parity, a.stage, dt, leader);
@@ -1227,8 +1289,7 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
     __syncthreads();
     if (threadIdx.x == 0 && ctl) {
       const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      atomicMax(&ctl->s[a.parity ^ 1].lambda,
-                static_cast<unsigned long long>(__double_as_longlong(mx)));
+      speed_update(ctl, a.parity ^ 1, mx);
     }
   }
   stage_epilogue(ctl, a.parity, a.stage, dt, leader);
@@ -1884,8 +1945,7 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     __syncthreads();
     if (threadIdx.x == 0 && ctl) {
       const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      atomicMax(&ctl->s[a.parity ^ 1].lambda,
-                static_cast<unsigned long long>(__double_as_longlong(mx)));
+      speed_update(ctl, a.parity ^ 1, mx);
     }
   }
   stage_epilogue(ctl, a.parity, a.stage, dt, leader);
@@ -2584,8 +2644,7 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
     __syncthreads();
     if (threadIdx.x == 0 && a.ctl) {
       const double mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-      atomicMax(&a.ctl->s[a.parity ^ 1].lambda,
-                static_cast<unsigned long long>(__double_as_longlong(mx)));
+      speed_update(a.ctl, a.parity ^ 1, mx);
     }
   }
   stage_epilogue(a.ctl, a.parity, a.stage, dt, leader);
