@@ -200,6 +200,25 @@ __device__ __forceinline__ void raise_error(RunCtl *ctl, int *err, int code) {
     atomicCAS(slot, 0, code);
 }
 
+// Appends `e` to a troubled-element list for every lane whose `want` is set:
+// one atomic per wave reserves the wave's entries, which follow the wave's
+// base in lane order, so the count is not updated once per element. Entries
+// past E are dropped. Every lane must be active.
+__device__ __forceinline__ void list_append(int *count, int *list, int E, bool want, int e) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0ull)
+    return;
+  const int first = __ffsll(static_cast<long long>(mask)) - 1;
+  int base = 0;
+  if ((static_cast<int>(threadIdx.x) & 63) == first)
+    base = atomicAdd(count, __popcll(mask));
+  base = __builtin_amdgcn_readlane(base, first);
+  const int below = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                              __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+  if (want && base + below < E)
+    list[base + below] = e;
+}
+
 // Speeds are non-negative doubles, which order like their bit patterns.
 __device__ __forceinline__ void speed_update(RunCtl *ctl, int parity, double speed) {
   atomicMax(speed_slots(ctl, parity) + (blockIdx.x % kSpeedSlots) * kSpeedStride,
@@ -874,24 +893,23 @@ __device__ __forceinline__ void euler_stage_body(const EulerMesh &mesh, const St
   }
   const double (&dens)[Np] = ys[0];
 
-  if (a.sensor && !a.residual_only && sub == 0 && valid) {
-    // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
-    double total = 0.0, high = 0.0;
+  if (a.sensor && !a.residual_only) {
+    bool troubled = false;
+    if (sub == 0 && valid) {
+      // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
+      double total = 0.0, high = 0.0;
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      total += dens[m] * dens[m];
-      if (m >= a.highest_begin)
-        high += dens[m] * dens[m];
+      for (int m = 0; m < Np; ++m) {
+        total += dens[m] * dens[m];
+        if (m >= a.highest_begin)
+          high += dens[m] * dens[m];
+      }
+      troubled = !(total == 0.0 || high / total <= a.threshold);
+      if (a.flags)
+        a.flags[e] = troubled ? 1 : 0;
     }
-    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
-    if (a.flags)
-      a.flags[e] = troubled ? 1 : 0;
-    if (troubled) {
-      // at most one entry per element and stage (the leader clears the count)
-      const int slot = atomicAdd(&a.counters[a.stage], 1);
-      if (slot < E)
-        a.list[slot] = e;
-    }
+    // at most one entry per element and stage (the leader clears the count)
+    list_append(&a.counters[a.stage], a.list, E, troubled, e);
   }
 
   if (a.want_lambda && !a.residual_only) {
@@ -1247,24 +1265,23 @@ __device__ __forceinline__ void euler_stage_vs_body(const EulerMesh mesh, const 
   const double o2 = dpp<quad_bcast<2>()>(y[0]);
   const double o3 = dpp<quad_bcast<3>()>(y[0]);
 
-  if (a.sensor && !a.residual_only && v == 0 && h == 0 && valid) {
-    // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
-    double total = 0.0, high = 0.0;
+  if (a.sensor && !a.residual_only) {
+    bool troubled = false;
+    if (v == 0 && h == 0 && valid) {
+      // Modal-decay sensor on density (euler_modal_limiter.cpp:18-24).
+      double total = 0.0, high = 0.0;
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      total += y[m] * y[m];
-      if (m >= a.highest_begin)
-        high += y[m] * y[m];
+      for (int m = 0; m < Np; ++m) {
+        total += y[m] * y[m];
+        if (m >= a.highest_begin)
+          high += y[m] * y[m];
+      }
+      troubled = !(total == 0.0 || high / total <= a.threshold);
+      if (a.flags)
+        a.flags[e] = troubled ? 1 : 0;
     }
-    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
-    if (a.flags)
-      a.flags[e] = troubled ? 1 : 0;
-    if (troubled) {
-      // at most one entry per element and stage (the leader clears the count)
-      const int slot = atomicAdd(&a.counters[a.stage], 1);
-      if (slot < E)
-        a.list[slot] = e;
-    }
+    // at most one entry per element and stage (the leader clears the count)
+    list_append(&a.counters[a.stage], a.list, E, troubled, e);
   }
 
   if (a.want_lambda && !a.residual_only) {
@@ -1910,17 +1927,14 @@ __device__ __forceinline__ void mx_stage(const EulerMesh mesh, const StageArgs a
     high += __shfl_xor(high, 16, 64);
     total += __shfl_xor(total, 32, 64);
     high += __shfl_xor(high, 32, 64);
+    bool troubled = false;
     if (grp == 0 && valid) {
-      const bool troubled = !(total == 0.0 || high / total <= a.threshold);
+      troubled = !(total == 0.0 || high / total <= a.threshold);
       if (a.flags)
         a.flags[e] = troubled ? 1 : 0;
-      if (troubled) {
-        // at most one entry per element and stage (the leader clears the count)
-        const int slot = atomicAdd(&a.counters[a.stage], 1);
-        if (slot < E)
-          a.list[slot] = e;
-      }
     }
+    // at most one entry per element and stage (the leader clears the count)
+    list_append(&a.counters[a.stage], a.list, E, troubled, e);
   }
 
   if (a.want_lambda && !a.residual_only) {
@@ -2604,23 +2618,22 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
       if (v == 0)
         dens[m] = y;
     }
-  if (a.sensor && !a.residual_only && valid) {
-    // modal-decay sensor on density (euler_modal_limiter.cpp:18-24)
-    double total = 0.0, high = 0.0;
+  if (a.sensor && !a.residual_only) {
+    bool troubled = false;
+    if (valid) {
+      // modal-decay sensor on density (euler_modal_limiter.cpp:18-24)
+      double total = 0.0, high = 0.0;
 #pragma unroll
-    for (int m = 0; m < Np; ++m) {
-      total += dens[m] * dens[m];
-      if (m >= a.highest_begin)
-        high += dens[m] * dens[m];
+      for (int m = 0; m < Np; ++m) {
+        total += dens[m] * dens[m];
+        if (m >= a.highest_begin)
+          high += dens[m] * dens[m];
+      }
+      troubled = !(total == 0.0 || high / total <= a.threshold);
+      if (a.flags)
+        a.flags[e] = troubled ? 1 : 0;
     }
-    const bool troubled = !(total == 0.0 || high / total <= a.threshold);
-    if (a.flags)
-      a.flags[e] = troubled ? 1 : 0;
-    if (troubled) {
-      const int slot = atomicAdd(&a.counters[a.stage], 1);
-      if (slot < E)
-        a.list[slot] = e;
-    }
+    list_append(&a.counters[a.stage], a.list, E, troubled, e);
   }
   if (a.want_lambda && !a.residual_only) {
     // |u_x| + c of the new cell average (cfl_timestep.cpp:33-47, 1D)
