@@ -680,7 +680,9 @@ template <int P> void launch_advect_p(const wasl_hip_engine &g, const whip::Stag
 }
 
 template <int P> void launch_euler_line_p(const wasl_hip_engine &g, const whip::StageArgs &a) {
-  const dim3 grid((g.E + 255) / 256);
+  // whip::kLineCells cells per wave, four waves per workgroup
+  const int waves = (g.E + whip::kLineCells - 1) / whip::kLineCells;
+  const dim3 grid((waves + 3) / 4);
   hipLaunchKernelGGL((whip::euler_line_stage<P>), grid, dim3(256), 0, g.stream,
                      g.euler_line_mesh(), a);
 }

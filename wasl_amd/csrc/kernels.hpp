@@ -2512,39 +2512,36 @@ __device__ __forceinline__ C4 flux_x_of(double g1, const C4 &s) {
   return {s.mx, s.mx * u + p, s.my * u, (s.E + p) * u};
 }
 
+// Cells a wave finalises: lanes 1-62. Lanes 0 and 63 hold the cells either
+// side (wrapped at periodic ends, clamped at outflow ends), so every face an
+// output cell needs is evaluated inside the wave.
+constexpr int kLineCells = 62;
+
 // One lane per element, state U[4][Np][E] (euler_line_residual,
 // dg/euler/line_residual.hpp:23-86, with line_euler_exterior,
 // src/boundaries/line_euler_exterior.cpp:13-43). The elements run left to
-// right, so the faces of element e are shared with e - 1 and e + 1; every
-// face's HLLC flux is evaluated as (left, right, +x) from the same traces on
-// both sides. Epilogue as in the triangle stage: RK combination, the density
-// sensor and the CFL speed |u_x| + c of the new cell average.
+// right, so the faces of element e are shared with e - 1 and e + 1. Each lane
+// evaluates the HLLC flux of its cell's left face, as (left, right, +x) from
+// the left neighbour's right trace (the previous lane's) and its own left
+// trace; its right face's flux is the next lane's. A wave covers kLineCells
+// cells and the halo lane at either end. Epilogue as in the triangle stage:
+// RK combination, the density sensor and the CFL speed |u_x| + c of the new
+// cell average.
 template <int P>
 __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const StageArgs a,
                                                 int block, bool leader, double *tab, double *red) {
   using D = LineE<P>;
   constexpr int Np = D::Np, NQ = D::NQ;
   const int E = mesh.E;
-  const int e_raw = block * 256 + static_cast<int>(threadIdx.x);
-  const bool valid = e_raw < E;
-  const int e = valid ? e_raw : E - 1;
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int wave = block * 4 + (static_cast<int>(threadIdx.x) >> 6);
+  const int e_raw = wave * kLineCells + lane - 1;
+  const bool valid = lane >= 1 && lane <= kLineCells && e_raw < E;
+  // the lane's cell: wrapped (periodic) or clamped (outflow) into [0, E)
+  const int e = mesh.periodic ? ((e_raw % E) + E) % E : (e_raw < 0 ? 0 : (e_raw < E ? e_raw : E - 1));
   const double g = mesh.gamma, g1 = g - 1.0;
   double c[4][Np];
   load_coefficients<Np>(a.uin, E, e, c);
-  // The neighbours' traces come from the adjacent lanes (the cells run left
-  // to right, one per lane). Only a wave's first lane (or the first cell) and
-  // its last lane (or the last cell) read a neighbour from memory, requested
-  // here with the own coefficients so that no second round trip follows.
-  const int lane = static_cast<int>(threadIdx.x) & 63;
-  const int en = e > 0 ? e - 1 : (mesh.periodic ? E - 1 : e);
-  const int ep = e < E - 1 ? e + 1 : (mesh.periodic ? 0 : e);
-  const bool far_l = lane == 0 || e == 0;
-  const bool far_r = lane == 63 || e_raw >= E - 1;
-  double cl[4][Np], cr[4][Np];
-  if (far_l)
-    load_coefficients<Np>(a.uin, E, en, cl);
-  if (far_r)
-    load_coefficients<Np>(a.uin, E, ep, cr);
   double dt = a.dt;
   if (!stage_prologue(a.ctl, a.parity, a.stage, dt, leader))
     return;
@@ -2572,20 +2569,20 @@ __device__ __forceinline__ void euler_line_body(const EulerLineDev mesh, const S
   }
   const double *bl = tab + D::kL, *br = tab + D::kR;
   const C4 own_l = trace_of<Np>(c, bl), own_r = trace_of<Np>(c, br);
-  // the left neighbour's right trace and the right neighbour's left trace,
-  // the same bits the neighbour evaluates for its own faces
-  C4 ext_l = {__shfl_up(own_r.r, 1, 64), __shfl_up(own_r.mx, 1, 64),
-              __shfl_up(own_r.my, 1, 64), __shfl_up(own_r.E, 1, 64)};
-  C4 ext_r = {__shfl_down(own_l.r, 1, 64), __shfl_down(own_l.mx, 1, 64),
-              __shfl_down(own_l.my, 1, 64), __shfl_down(own_l.E, 1, 64)};
-  // outflow ends: the exterior is the owner's own trace at that end
-  const bool open_l = e == 0 && !mesh.periodic, open_r = e == E - 1 && !mesh.periodic;
-  if (far_l)
-    ext_l = open_l ? own_l : trace_of<Np>(cl, br);
-  if (far_r)
-    ext_r = open_r ? own_r : trace_of<Np>(cr, bl);
-  const C4 fl = hllc(g, ext_l, own_l, 1.0, 0.0);
-  const C4 fr = hllc(g, own_r, ext_r, 1.0, 0.0);
+  // The left face: the previous lane's right trace against this lane's left
+  // trace. At outflow ends the exterior is the owner's own trace: cell 0's
+  // left face is (own_l, own_l), and the face right of cell E - 1, evaluated
+  // by the lane after it, is (its right trace, the same).
+  C4 L = {__shfl_up(own_r.r, 1, 64), __shfl_up(own_r.mx, 1, 64), __shfl_up(own_r.my, 1, 64),
+          __shfl_up(own_r.E, 1, 64)};
+  C4 R = own_l;
+  if (!mesh.periodic && e_raw == 0)
+    L = own_l;
+  if (!mesh.periodic && e_raw == E)
+    R = L;
+  const C4 fl = hllc(g, L, R, 1.0, 0.0);
+  const C4 fr = {__shfl_down(fl.r, 1, 64), __shfl_down(fl.mx, 1, 64), __shfl_down(fl.my, 1, 64),
+                 __shfl_down(fl.E, 1, 64)};
   const double vl[4] = {fl.r, fl.mx, fl.my, fl.E}, vr[4] = {fr.r, fr.mx, fr.my, fr.E};
   const double ij = mesh.inv_jac[e];
   // U^n first, all of it: a read issued after a store would wait for that
