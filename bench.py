@@ -43,7 +43,7 @@ MEASURED_TRAFFIC = {
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (224747.1 * 2.0 + 262158.2) * 1024.0,
     ("whip::advect_stage<1>", 1, 524288, "fp64"): (26869.3 * 2.0 + 12288.0) * 1024.0,
     ("whip::advect_stage<3>", 3, 524288, "fp64"): (51451.3 * 2.0 + 40960.0) * 1024.0,
-    ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (93865.0 * 2.0 + 98315.6) * 1024.0,
+    ("whip::euler_line_stage<2>", 2, 1048576, "fp64"): (94410.7 * 2.0 + 98319.2) * 1024.0,
 }
 
 
