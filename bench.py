@@ -38,7 +38,7 @@ FP64_PEAK_TFLOPS = 78.6
 # profiles/advect_stage_524k.md and profiles/euler_line_stage_1m.md.
 # Other workloads report null.
 MEASURED_TRAFFIC = {
-    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7824.1 * 2.0 + 6144.3) * 1024.0,
+    ("whip::euler_stage<2,4>", 2, 32768, "fp64"): (7827.3 * 2.0 + 6144.3) * 1024.0,
     ("whip::euler_stage<2,2>", 2, 32768, "fp64"): (7965.57 * 2.0 + 6146.72) * 1024.0,
     ("whip::euler_stage_mx<3>", 3, 524288, "mixed"): (224747.1 * 2.0 + 262158.2) * 1024.0,
     ("whip::advect_stage<1>", 1, 524288, "fp64"): (26869.3 * 2.0 + 12288.0) * 1024.0,
