@@ -101,6 +101,32 @@ def test_sod_tutorial_accuracy_gate(gpu):
     assert errs[0] / errs[1] >= 1.6 and errs[1] / errs[2] >= 1.6 and errs[2] <= 4e-3
 
 
+@pytest.mark.parametrize("cells", [2, 61, 62, 63, 124, 125, 187])
+@pytest.mark.parametrize("bc", [O.PERIODIC, O.OUTFLOW])
+def test_wave_ends(gpu, cells, bc):
+    """Cell counts either side of the 62 cells a wave of the stage kernel
+    finalises (its first and last lanes hold the cells beyond them, wrapped
+    at periodic ends and clamped at outflow ends): residual, CFL dt and two
+    SSP-RK3 steps against the oracle at P1 and P3, on the irregular mesh
+    whose elements are listed in reverse with alternating orientation."""
+    mesh = RM.irregular_line(cells)
+    for p in (1, 3):
+        g = _engine(gpu, mesh, p, bc)
+        o = O.line_euler(*mesh, p, left=bc, right=bc)
+        U = o.project(O.IC_EULER_REGIONS, K.SHOCK_TUBE)
+        err = K.rel_inf_per_variable(g.residual(U), o.residual(U), 4)
+        print(f"{cells} cells P{p} bc {bc}: residual rel {err:.2e}", flush=True)
+        assert err <= K.STEP_TOL
+        g.state = U
+        ref = U
+        for k in range(1, 3):
+            dt = o.stable_dt(ref, 0.2)
+            assert abs(g.stable_time_step(0.2) / dt - 1.0) <= 1e-15
+            g.step(dt)
+            ref = o.step(ref, dt)
+            assert K.rel_inf_per_variable(g.state, ref, 4) <= K.STEP_TOL * k
+
+
 def test_edge_cases(gpu):
     mesh = RM.irregular_line(12)
     with pytest.raises(gpu.WaslHipError, match="inflow is not implemented"):
